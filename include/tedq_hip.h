@@ -1,0 +1,118 @@
+/*
+ * tedq_hip.h — C ABI of the MI355X (gfx950) execution engine for TeD-Q compiled circuits.
+ *
+ * The library (ted-q_amd/tedq_amd/_lib/libtedq_hip.so) is bound from Python
+ * with ctypes (ted-q_amd/tedq_amd/engine/hip.py).  Conventions for every entry point:
+ *   - pointers named dev_* / state / psi / lam / mats / partials / out are DEVICE
+ *     pointers allocated by the caller (torch's caching allocator); the library
+ *     never allocates or frees device memory;
+ *   - pointers documented as "host" are read on the host during the call;
+ *   - `stream` is a hipStream_t (the caller's current stream); all work is
+ *     enqueued asynchronously and no entry point synchronises;
+ *   - the return value is 0 on success, negative on error; tedq_last_error()
+ *     returns the message of the last failure on the calling thread.
+ * State layout: a batch of circuits is [batch][2^n] complex64 (interleaved
+ * float re/im), qubit q of a circuit is flat bit n-1-q (the reference's
+ * C-order [2]*n tensor, tedq/backends/pytorch_backend.py:513-522).
+ */
+#ifndef TEDQ_HIP_H
+#define TEDQ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library plumbing ---------------------------------------------------- */
+const char* tedq_last_error(void);
+int tedq_abi_version(void);
+/* props[0..3] = {CU count, max LDS bytes per workgroup, wavefront size, arch major*100+minor} */
+int tedq_device_info(int* props /* host, 4 ints */);
+
+/* ---- state preparation --------------------------------------------------- */
+/* |0...0> for every circuit.  Replaces default_initstate
+ * (tedq/backends/pytorch_backend.py:513-522). */
+int tedq_sv_init_zero(void* state, int64_t batch, int n_qubits, void* stream);
+/* Broadcast one user state (dev, 2^n complex64) to every circuit.  Replaces
+ * get_initstate with an IintStateVector (pytorch_backend.py:500-510). */
+int tedq_sv_init_copy(void* state, const void* init, int64_t batch, int n_qubits, void* stream);
+
+/* ---- gate tables ----------------------------------------------------------- */
+/* Per (circuit, slot): fused gate matrix (complex64) and, when amats != NULL,
+ * per trainable parameter the fp64 matrix A with dL/dtheta = 2 Re sum A_ij C_ij.
+ * params: dev [batch][n_params] float32 (params_are_f64 = 0) or float64.
+ * slot_recs / comp_recs: dev int32 records (8 words each, csrc/sv_tables.hip);
+ * const_pool: dev complex128 constants; err_flag: dev int, OR-ed with 1 on an
+ * unknown gate code.  Replaces the per-gate tensor builders
+ * (pytorch_backend.py:579-1188) and _parser_circuit (compiled_circuit.py:418-440). */
+int tedq_sv_build_tables(const void* params, int params_are_f64, int64_t batch, int n_params,
+                         const void* slot_recs, int n_slots, const void* comp_recs, const void* const_pool,
+                         void* mats, int64_t mat_stride, void* amats, int* err_flag, void* stream);
+
+/* ---- fused passes ---------------------------------------------------------- */
+/* One forward pass (a tile of 2^tile_bits amplitudes per workgroup, register
+ * stages of 2^reg_bits amplitudes per thread) over every circuit; dev_prog is
+ * the pass program (int32 words, csrc/sv_passes.hip), tile_mask its physical
+ * tile bits.  A sequence of passes replaces the SV propagation loop
+ * tensordot + permute per gate (pytorch_backend.py:358-380). */
+int tedq_sv_apply_pass(void* state, int64_t batch, int n_qubits, int tile_bits, int reg_bits, int threads_log2,
+                       const int* dev_prog, uint64_t tile_mask, const void* mats, int64_t mat_stride,
+                       void* stream);
+/* One adjoint pass: applies the daggered gates of a pass in reverse order to
+ * psi and lam and writes the gradient bilinears of each parametrised slot to
+ * partials [batch][n_gslots][partial_rows][8] (float).  Replaces torch
+ * autograd through the SV loop (the default back_prop method,
+ * pytorch_backend.py:142-144). */
+int tedq_sv_adjoint_pass(void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits, int reg_bits,
+                         int threads_log2, const int* dev_prog, uint64_t tile_mask, const void* mats,
+                         int64_t mat_stride, float* partials, int64_t partial_rows, int n_gslots, void* stream);
+/* grad[b][p] = 2 Re sum_ij A[b][p][ij] C[b][g(p)][ij] with C reduced in fp64
+ * (compensated) from the partial rows; grad is float32 or float64
+ * (grad_is_f64).  pgslot: dev int32 [n_params]; C: dev double scratch
+ * [batch][n_gslots][8]. */
+int tedq_sv_grad_reduce(const float* partials, int64_t batch, int n_gslots, int64_t partial_rows,
+                        const void* amats, const int* pgslot, int n_params, double* C, void* grad,
+                        int grad_is_f64, void* stream);
+
+/* ---- measurements ------------------------------------------------------------- */
+/* Number of reduction chunks (rows of every `partial` buffer below) for n qubits. */
+int64_t tedq_sv_reduce_chunks(int n_qubits);
+/* out[b][m] = <psi_b| Z-string(mask_m) |psi_b> in fp64, all masks in one pass,
+ * chunks combined with Neumaier summation; mask 0 is the norm.  n_masks <= 32.
+ * partial: dev double [batch][chunks][n_masks].  Replaces the expectation
+ * branch of get_measurement_results for PauliZ/I observables
+ * (pytorch_backend.py:399-459). */
+int tedq_sv_expval_z(const void* state, int64_t batch, int n_qubits, const uint64_t* dev_masks, int n_masks,
+                     double* partial, double* out, void* stream);
+/* lam = (sum_m dy[b][m] Z-string_m) psi; n_masks may be 0 (lam = 0).  dy: dev double [batch][n_masks]. */
+int tedq_sv_lambda_z(const void* psi, void* lam, int64_t batch, int n_qubits, const uint64_t* dev_masks,
+                     int n_masks, const double* dy, void* stream);
+/* Re <psi| Zsign(zmask) (F_0 x ... x F_{q-1}) |psi> for q <= 4 single-qubit
+ * factors; bits: host int32[q] physical positions, factors: host float
+ * [q][2][2][re,im].  Replaces the non-diagonal expectation branch
+ * (pytorch_backend.py:402-424, 428-459). */
+int tedq_sv_expval_factored(const void* state, int64_t batch, int n_qubits, int q, const int* bits,
+                            const float* factors, uint64_t zmask, double* partial, double* out, void* stream);
+/* lam += coef[b] * Zsign * ((F + F^dagger)/2) psi  (coef: dev double [batch]). */
+int tedq_sv_lambda_factored(const void* psi, void* lam, int64_t batch, int n_qubits, int q, const int* bits,
+                            const float* factors, uint64_t zmask, const double* coef, void* stream);
+/* Probabilities: bits == NULL and k == n gives |psi|^2; otherwise marginals over
+ * the k <= 10 physical bits listed (host int32, most significant output bit
+ * first).  out: dev float [batch][2^k].  Replaces the probs branch
+ * (pytorch_backend.py:461-472). */
+int tedq_sv_probs(const void* state, int64_t batch, int n_qubits, int k, const int* bits, double* partial,
+                  float* out, void* stream);
+/* lam += g[b][marg(x)] psi_x (gradient seed of a probability output; g: dev float). */
+int tedq_sv_lambda_probs(const void* psi, void* lam, int64_t batch, int n_qubits, int k, const int* bits,
+                         const float* g, void* stream);
+/* lam += scale * G (gradient seed of a state output; G: dev complex64). */
+int tedq_sv_axpy(void* lam, const void* G, int64_t batch, int n_qubits, float scale, void* stream);
+/* out[b][m] = compensated sum over chunks of partial[b][chunk][m]. */
+int tedq_sum_partials(const double* partial, int64_t batch, int64_t nchunk, int M, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TEDQ_HIP_H */

@@ -1,0 +1,272 @@
+"""Fusion planner: gate list -> device passes of LDS-staged tiles.
+
+State layout (the reference's, pytorch_backend.py:513-522 with
+compiled_circuit.py:134-139): amplitudes of one circuit are a contiguous
+C-order [2]*n complex64 tensor, so qubit q is flat bit p = n-1-q and a batch of
+circuits is [B][2^n].
+
+A *pass* is one launch of the block kernel.  It owns a set of k physical bits
+(the *tile* bits, always including the lowest bits so that global loads are
+contiguous runs); every workgroup loads one 2^k tile per circuit into LDS,
+applies all gates of the pass that act inside the tile, and writes it back
+once.  Inside a pass, gates are grouped into *stages*: each stage names r tile
+bits that are held in registers (R = 2^r amplitudes per thread), so every
+gate of the stage is pure VALU work on registers; stages are separated by one
+LDS gather/scatter.
+
+Pass selection is a greedy topological sweep: a gate may be scheduled once it
+is the next pending gate on all its qubits, so gates on disjoint qubits are
+freely reordered (they commute) and the CNOT chains of hardware-efficient
+ansatze fold into about two passes per layer.
+"""
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Op codes shared with csrc/sv_passes.hip (kOp* constants there).
+OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_U2, OP_GRAD1, OP_CGRAD1, OP_CD1 = 1, 2, 3, 4, 5, 6, 7, 8, 9
+OP_DAGGER = 1 << 8
+
+# Slot kinds shared with csrc/sv_tables.hip.
+SLOT_1Q, SLOT_2Q = 0, 1
+
+# Component sources: a parameter is either a trainable column of the params
+# row (>= 0) or a constant in the fp64 constant pool (encoded -1 - index).
+
+_DIAGONAL_CODES = {0, 4, 5, 6, 16, 18}          # I, Z, S, T, RZ, PhaseShift
+_ONEQ_CODES = {0, 1, 2, 3, 4, 5, 6, 7, 14, 15, 16, 17, 18, 23}
+_PARAM_COUNT = {14: 1, 15: 1, 16: 1, 17: 3, 18: 1}
+
+
+@dataclass
+class Component:
+    """One gate factor of a fused slot: its code, dagger flag and parameter sources."""
+    code: int
+    dagger: bool
+    srcs: tuple            # parameter sources (see module note), up to 3
+    const_mat: int = -1    # constant-pool offset of a user matrix (Unitary) or -1
+
+
+@dataclass
+class Slot:
+    """A device matrix: product of components (1q, 2x2) or one 4x4 user unitary."""
+    kind: int
+    comps: list
+    mat_off: int = 0       # complex64 offset inside one batch row of the table
+    gslot: int = -1        # gradient slot index or -1
+
+    @property
+    def dim(self):
+        return 2 if self.kind == SLOT_1Q else 4
+
+    @property
+    def trainable(self):
+        return [s for c in self.comps for s in c.srcs if s >= 0]
+
+    @property
+    def diagonal(self):
+        return self.kind == SLOT_1Q and all(c.code in _DIAGONAL_CODES for c in self.comps)
+
+
+@dataclass
+class Unit:
+    """A gate as the kernels see it: physical bits it touches and how to apply it."""
+    kind: str              # "u1", "cu1", "cx", "cz", "u2"
+    bits: tuple            # for u1/cu1/cx: (target, *controls); u2: (msb, lsb); cz: all bits
+    comp: Component = None
+    index: int = 0
+
+
+@dataclass
+class Stage:
+    reg_bits: list         # local tile-bit indices held in registers (len r)
+    ops: list = field(default_factory=list)   # tuples (type, b0, b1, cmask, slot_index)
+
+
+@dataclass
+class Pass:
+    tile_bits: list        # physical bit positions, ascending (len k)
+    stages: list = field(default_factory=list)
+
+
+@dataclass
+class Plan:
+    n: int
+    k: int
+    r: int
+    passes: list
+    slots: list
+    const_pool: np.ndarray
+    n_params: int
+    n_gslots: int
+    mat_stride: int        # complex64 entries per batch row of the slot table
+
+
+def gate_to_units(n, gate, srcs, const_pool):
+    """Lower one traced gate to kernel units.  ``srcs`` are its parameter sources."""
+    code, qubits = gate.gate_code, list(gate.qubits)
+    bits = [n - 1 - q for q in qubits]
+    dagger = bool(getattr(gate, "is_inverse", False))
+    name = gate.name[:-4] if dagger else gate.name
+
+    def comp(c, s=(), mat=-1):
+        return Component(c, dagger, tuple(s), mat)
+
+    if code in _ONEQ_CODES and len(bits) == 1:
+        if code == 23:
+            off = len(const_pool)
+            const_pool.extend(np.asarray(gate.matrix, dtype=np.complex128).reshape(-1).tolist())
+            return [Unit("u1", (bits[0],), comp(23, (), off))]
+        return [Unit("u1", (bits[0],), comp(code, srcs))]
+    if name == "CNOT":
+        return [Unit("cx", (bits[1], bits[0]))]
+    if name == "Toffoli":
+        return [Unit("cx", (bits[2], bits[0], bits[1]))]
+    if name == "CZ":
+        return [Unit("cz", (bits[0], bits[1]))]
+    if name == "SWAP":
+        a, b = bits
+        return [Unit("cx", (b, a)), Unit("cx", (a, b)), Unit("cx", (b, a))]
+    if name == "CSWAP":
+        c, a, b = bits
+        return [Unit("cx", (a, b)), Unit("cx", (b, c, a)), Unit("cx", (a, b))]
+    if name == "CY":
+        return [Unit("cu1", (bits[1], bits[0]), comp(3))]
+    if name in ("CRX", "CRY", "CRZ", "ControlledPhaseShift"):
+        block = {"CRX": 14, "CRY": 15, "CRZ": 16, "ControlledPhaseShift": 18}[name]
+        return [Unit("cu1", (bits[1], bits[0]), comp(block, srcs))]
+    if code == 23 and len(bits) == 2:
+        off = len(const_pool)
+        const_pool.extend(np.asarray(gate.matrix, dtype=np.complex128).reshape(-1).tolist())
+        return [Unit("u2", (bits[0], bits[1]), comp(23, (), off))]
+    raise NotImplementedError(f"{gate.name} on {len(bits)} qubits is not supported by the SV engine")
+
+
+def _schedule(units, order, max_bits, seed_bits, allowed_bits=None):
+    """Greedy topological sweep.
+
+    ``order`` lists candidate unit indices in program order; returns the
+    scheduled indices (a valid order) and the chosen bit set.  A unit is ready
+    when it heads the pending queue of all its bits.
+    """
+    pending = {}
+    for ui in order:
+        for b in units[ui].bits:
+            pending.setdefault(b, []).append(ui)
+    heads = {b: 0 for b in pending}
+    chosen = set(seed_bits)
+    done = []
+    remaining = list(order)
+    progress = True
+    while progress:
+        progress = False
+        nxt = []
+        for ui in remaining:
+            u = units[ui]
+            if any(pending[b][heads[b]] != ui for b in u.bits):
+                nxt.append(ui)
+                continue
+            ub = set(u.bits)
+            if allowed_bits is not None and not ub <= allowed_bits:
+                nxt.append(ui)
+                continue
+            if not ub <= chosen:
+                if len(chosen | ub) > max_bits:
+                    nxt.append(ui)
+                    continue
+                chosen |= ub
+            for b in u.bits:
+                heads[b] += 1
+            done.append(ui)
+            progress = True
+        remaining = nxt
+    return done, chosen
+
+
+def build_plan(n, units, n_params, const_pool, k_max=12, r=4, coalesce_bits=3):
+    """Partition ``units`` (program order) into passes and register stages."""
+    k = min(n, k_max)
+    widest = max((len(u.bits) for u in units), default=1)
+    if widest > k:
+        raise ValueError(f"a gate acts on {widest} qubits but tiles hold only {k}")
+    r = min(k, max(r, widest))
+    slots = []
+    passes = []
+    todo = list(range(len(units)))
+    while todo:
+        # seed the tile with the lowest bits (contiguous global runs); shrink the
+        # seed when it leaves no room for the next gate
+        for c in range(min(n, coalesce_bits), -1, -1):
+            done, chosen = _schedule(units, todo, k, list(range(c)))
+            if done:
+                break
+        if not done:
+            raise RuntimeError("planner made no progress")
+        for b in range(n):
+            if len(chosen) >= k:
+                break
+            chosen.add(b)
+        tile_bits = sorted(chosen)
+        done_set = set(done)
+        todo = [u for u in todo if u not in done_set]
+        passes.append(_build_pass(units, done, tile_bits, r, slots))
+    mat_off = 0
+    gslot = 0
+    for s in slots:
+        s.mat_off = mat_off
+        mat_off += s.dim * s.dim
+        if s.trainable:
+            s.gslot = gslot
+            gslot += 1
+    return Plan(n=n, k=k, r=r, passes=passes, slots=slots,
+                const_pool=np.asarray(const_pool, dtype=np.complex128),
+                n_params=n_params, n_gslots=gslot, mat_stride=max(mat_off, 1))
+
+
+def _build_pass(units, pass_units, tile_bits, r, slots):
+    local = {b: i for i, b in enumerate(tile_bits)}
+    allowed = set(tile_bits)
+    p = Pass(tile_bits=tile_bits)
+    todo = list(pass_units)
+    while todo:
+        done, chosen = _schedule(units, todo, r, [], allowed)
+        if not done:
+            raise RuntimeError("stage planner made no progress")
+        for b in tile_bits:
+            if len(chosen) >= r:
+                break
+            chosen.add(b)
+        reg_phys = sorted(chosen)
+        reg_of = {b: i for i, b in enumerate(reg_phys)}
+        stage = Stage(reg_bits=[local[b] for b in reg_phys])
+        last_on_bit = {}
+        for ui in done:
+            u = units[ui]
+            if u.kind == "u1":
+                b = u.bits[0]
+                prev = last_on_bit.get(b)
+                if prev is not None and stage.ops[prev][0] == "u1":
+                    slots[stage.ops[prev][4]].comps.append(u.comp)
+                    continue
+                slots.append(Slot(SLOT_1Q, [u.comp]))
+                stage.ops.append(("u1", reg_of[b], 0, 0, len(slots) - 1))
+            elif u.kind == "cu1":
+                cmask = sum(1 << reg_of[c] for c in u.bits[1:])
+                slots.append(Slot(SLOT_1Q, [u.comp]))
+                stage.ops.append(("cu1", reg_of[u.bits[0]], 0, cmask, len(slots) - 1))
+            elif u.kind == "cx":
+                cmask = sum(1 << reg_of[c] for c in u.bits[1:])
+                stage.ops.append(("cx", reg_of[u.bits[0]], 0, cmask, -1))
+            elif u.kind == "cz":
+                mask = sum(1 << reg_of[c] for c in u.bits)
+                stage.ops.append(("cz", 0, 0, mask, -1))
+            elif u.kind == "u2":
+                slots.append(Slot(SLOT_2Q, [u.comp]))
+                stage.ops.append(("u2", reg_of[u.bits[0]], reg_of[u.bits[1]], 0, len(slots) - 1))
+            for b in u.bits:
+                last_on_bit[b] = len(stage.ops) - 1
+        p.stages.append(stage)
+        done_set = set(done)
+        todo = [u for u in todo if u not in done_set]
+    return p

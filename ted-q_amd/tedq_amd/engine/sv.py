@@ -1,0 +1,305 @@
+"""State-vector executor: one compiled circuit structure, any batch of parameter rows.
+
+Forward (replaces tedq/backends/pytorch_backend.py:358-391):
+    tables  <- gate matrices of every fused slot for every parameter row (fp64 -> c64)
+    psi     <- |0..0> (or the traced initial state) for every row
+    psi     <- pass_P ... pass_1 psi              (one fused LDS-tiled kernel per pass)
+    results <- FP64 compensated reductions (all Z-string observables in one pass),
+               factored observables, probabilities, or the state itself.
+Backward (replaces torch autograd through the tensordot/permute graph):
+    lambda  <- O_eff psi with O_eff = sum_m dy_m O_m (+ probability/state seeds)
+    for each pass in reverse: one adjoint kernel walks psi and lambda backwards,
+               reducing the bilinears C_ij of every parametrised slot,
+    grad    <- 2 Re sum_ij A_ij C_ij per parameter (fp64).
+"""
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import hip
+from .planner import build_plan
+from .program import encode_plan
+
+_Z = np.array([[1, 0], [0, -1]], dtype=np.complex128)
+_I = np.eye(2, dtype=np.complex128)
+
+
+@dataclass
+class MeasSpec:
+    """How one measurement is produced on the device."""
+    kind: str                  # "zexp", "fexp", "probs", "state"
+    shape: tuple               # per-circuit output shape
+    zindex: int = -1           # column in the Z-string launch
+    zmask: int = 0
+    fbits: list = field(default_factory=list)     # factored observable: physical bits
+    ffactors: np.ndarray = None                    # [q, 2, 2] complex
+    pbits: list = field(default_factory=list)     # probability bits, MSB first (empty: all)
+
+
+def _observable_factors(n, meas):
+    """Per-physical-bit 2x2 factor of an expval observable (reference order of application)."""
+    obs = meas.obs if isinstance(meas.obs, list) else [meas.obs]
+    factors = {}
+    for ob in obs:
+        if len(ob.qubits) != 1:
+            raise NotImplementedError("multi-qubit observables are not supported by the SV engine")
+        bit = n - 1 - ob.qubits[0]
+        m = np.asarray(ob.matrix, dtype=np.complex128).reshape(2, 2)
+        factors[bit] = m @ factors.get(bit, _I)
+    return factors
+
+
+def compile_measurements(n, measurements):
+    from ..measurement import Expectation, Probability, State  # pylint: disable=import-outside-toplevel
+    specs, zmasks = [], []
+    for meas in measurements:
+        if meas.return_type is Expectation:
+            factors = _observable_factors(n, meas)
+            zmask, rest = 0, {}
+            for bit, f in factors.items():
+                if np.array_equal(f, _I):
+                    continue
+                if np.array_equal(f, _Z):
+                    zmask |= 1 << bit
+                else:
+                    rest[bit] = f
+            if not rest:
+                specs.append(MeasSpec("zexp", (), zindex=len(zmasks), zmask=zmask))
+                zmasks.append(zmask)
+            else:
+                if len(rest) > 4:
+                    raise NotImplementedError("expval over more than 4 non-Z qubits is not supported")
+                bits = sorted(rest)
+                specs.append(MeasSpec("fexp", (), zmask=zmask, fbits=bits,
+                                      ffactors=np.stack([rest[b] for b in bits])))
+        elif meas.return_type is Probability:
+            if meas.qubits is None:
+                specs.append(MeasSpec("probs", tuple([2] * n)))
+            else:
+                qs = sorted(set(meas.qubits))
+                specs.append(MeasSpec("probs", tuple([2] * len(qs)), pbits=[n - 1 - q for q in qs]))
+        elif meas.return_type is State:
+            specs.append(MeasSpec("state", tuple([2] * n)))
+        else:
+            raise NotImplementedError(f"measurement {meas.return_type} is not supported")
+    if len(zmasks) > 32:
+        raise NotImplementedError("more than 32 Z-string expectation values per circuit")
+    return specs, zmasks
+
+
+class _DeviceBuffers:
+    def __init__(self, enc, zmasks, device):
+        self.prog = torch.from_numpy(enc.prog).to(device)
+        self.slot_recs = torch.from_numpy(enc.slot_recs.reshape(-1)).to(device)
+        self.comp_recs = torch.from_numpy(enc.comp_recs.reshape(-1)).to(device)
+        self.const_pool = torch.from_numpy(enc.const_pool.reshape(-1)).to(device)
+        self.pgslot = torch.from_numpy(enc.pgslot).to(device)
+        self.zmasks = torch.tensor(np.asarray(zmasks, dtype=np.uint64).view(np.int64), device=device) \
+            if zmasks else None
+        self.prog_base = self.prog.data_ptr()
+
+    def pass_ptr(self, word_offset):
+        import ctypes  # pylint: disable=import-outside-toplevel
+        return ctypes.c_void_p(self.prog_base + 4 * word_offset)
+
+
+class SVExecutor:
+    """Compiled SV program for one circuit structure."""
+
+    def __init__(self, n, units, n_params, const_pool, measurements, init_state=None, k_max=12, r=4):
+        self.n = n
+        self.n_params = n_params
+        self.plan = build_plan(n, units, n_params, const_pool, k_max=k_max, r=r)
+        self.enc = encode_plan(self.plan)
+        self.specs, self.zmasks = compile_measurements(n, measurements)
+        self.init_state = None
+        if init_state is not None:
+            flat = np.asarray(init_state, dtype=np.complex64).reshape(-1)
+            if flat.size != 2 ** n:
+                raise ValueError(f"initial state has {flat.size} amplitudes, expected {2 ** n}")
+            self.init_state = flat
+        self._dev = {}
+        # optional timing probe with begin(kind)/end(kind) around every pass launch
+        self.probe = None
+
+    # ------------------------------------------------------------------
+    @property
+    def n_passes(self):
+        return len(self.plan.passes)
+
+    def _buffers(self, device):
+        key = (device.type, device.index)
+        if key not in self._dev:
+            buf = _DeviceBuffers(self.enc, self.zmasks, device)
+            if self.init_state is not None:
+                buf.init = torch.from_numpy(self.init_state.view(np.float32).copy()).to(device)
+            self._dev[key] = buf
+        return self._dev[key]
+
+    def _threads_log2(self, adjoint=False):
+        k, r = self.plan.k, self.plan.r
+        tT = k - r
+        if k < self.n:
+            return tT
+        cap = 8 if (not adjoint or r <= 4) else 7
+        return max(tT, min(cap, max(6, tT)))
+
+    def partial_rows(self):
+        k, r = self.plan.k, self.plan.r
+        tpt = 1 << (k - r)
+        w = max(1, tpt // 64)
+        return (1 << (self.n - k)) * w
+
+    # ------------------------------------------------------------------
+    def build_tables(self, params, need_grad):
+        """params: [B, P] float32/float64 on the device."""
+        dev = params.device
+        buf = self._buffers(dev)
+        B = params.shape[0]
+        mats = torch.empty((B, self.plan.mat_stride, 2), dtype=torch.float32, device=dev)
+        amats = torch.zeros((B, max(self.n_params, 1), 4, 2), dtype=torch.float64, device=dev) if need_grad else None
+        err = torch.zeros(1, dtype=torch.int32, device=dev)
+        params = params.contiguous()
+        hip.call("tedq_sv_build_tables", hip.ptr(params) if self.n_params else None,
+                 int(params.dtype == torch.float64), B, self.n_params, hip.ptr(buf.slot_recs),
+                 len(self.plan.slots), hip.ptr(buf.comp_recs), hip.ptr(buf.const_pool), hip.ptr(mats),
+                 self.plan.mat_stride, hip.ptr(amats), hip.ptr(err), hip.stream_handle(dev))
+        return mats, amats
+
+    def run_forward(self, params, need_grad=False):
+        """Apply the circuit to every parameter row.  Returns (psi [B, 2^n] complex64, mats, amats)."""
+        dev = params.device
+        buf = self._buffers(dev)
+        B = params.shape[0]
+        mats, amats = self.build_tables(params, need_grad)
+        psi = torch.empty((B, 2 ** self.n), dtype=torch.complex64, device=dev)
+        st = hip.stream_handle(dev)
+        if self.init_state is None:
+            hip.call("tedq_sv_init_zero", hip.ptr(psi), B, self.n, st)
+        else:
+            hip.call("tedq_sv_init_copy", hip.ptr(psi), hip.ptr(buf.init), B, self.n, st)
+        tl = self._threads_log2()
+        probe = self.probe
+        for off, mask in zip(self.enc.fwd_offsets, self.enc.tile_masks):
+            if probe:
+                probe.begin("sv_pass_fwd")
+            hip.call("tedq_sv_apply_pass", hip.ptr(psi), B, self.n, self.plan.k, self.plan.r, tl,
+                     buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, st)
+            if probe:
+                probe.end("sv_pass_fwd")
+        return psi, mats, amats
+
+    def measure(self, psi):
+        """List of per-measurement outputs, each [B, *shape] (expvals fp64, probs fp32, state c64)."""
+        dev = psi.device
+        buf = self._buffers(dev)
+        B = psi.shape[0]
+        st = hip.stream_handle(dev)
+        nchunk = hip.load().tedq_sv_reduce_chunks(self.n)
+        zout = None
+        if self.zmasks:
+            M = len(self.zmasks)
+            partial = torch.empty((B, nchunk, M), dtype=torch.float64, device=dev)
+            zout = torch.empty((B, M), dtype=torch.float64, device=dev)
+            hip.call("tedq_sv_expval_z", hip.ptr(psi), B, self.n, hip.ptr(buf.zmasks), M, hip.ptr(partial),
+                     hip.ptr(zout), st)
+        outs = []
+        for spec in self.specs:
+            if spec.kind == "zexp":
+                outs.append(zout[:, spec.zindex])
+            elif spec.kind == "fexp":
+                bits, fac = _factored_args(spec, dev)
+                partial = torch.empty((B, nchunk), dtype=torch.float64, device=dev)
+                out = torch.empty((B,), dtype=torch.float64, device=dev)
+                hip.call("tedq_sv_expval_factored", hip.ptr(psi), B, self.n, len(spec.fbits), _hptr(bits),
+                         _hptr(fac), spec.zmask, hip.ptr(partial), hip.ptr(out), st)
+                outs.append(out)
+            elif spec.kind == "probs":
+                k = len(spec.pbits) if spec.pbits else self.n
+                out = torch.empty((B, 2 ** k), dtype=torch.float32, device=dev)
+                if spec.pbits:
+                    bits = np.ascontiguousarray(spec.pbits, dtype=np.int32)
+                    partial = torch.empty((B, nchunk, 2 ** k), dtype=torch.float64, device=dev)
+                    hip.call("tedq_sv_probs", hip.ptr(psi), B, self.n, k, _hptr(bits), hip.ptr(partial),
+                             hip.ptr(out), st)
+                else:
+                    hip.call("tedq_sv_probs", hip.ptr(psi), B, self.n, k, None, None, hip.ptr(out), st)
+                outs.append(out.view((B,) + spec.shape))
+            else:
+                outs.append(psi.view((B,) + spec.shape))
+        return outs
+
+    # ------------------------------------------------------------------
+    def seed_adjoint(self, psi, grads):
+        """lambda = sum_m (d out_m)^* seed_m applied to psi; grads aligned with self.specs (None = zero)."""
+        dev = psi.device
+        buf = self._buffers(dev)
+        B = psi.shape[0]
+        st = hip.stream_handle(dev)
+        lam = torch.empty_like(psi)
+        M = len(self.zmasks)
+        dy = None
+        if M:
+            dy = torch.zeros((B, M), dtype=torch.float64, device=dev)
+            for spec, g in zip(self.specs, grads):
+                if spec.kind == "zexp" and g is not None:
+                    dy[:, spec.zindex] += g.to(torch.float64).reshape(B)
+        hip.call("tedq_sv_lambda_z", hip.ptr(psi), hip.ptr(lam), B, self.n,
+                 hip.ptr(buf.zmasks) if M else None, M, hip.ptr(dy), st)
+        for spec, g in zip(self.specs, grads):
+            if g is None or spec.kind == "zexp":
+                continue
+            if spec.kind == "fexp":
+                bits, fac = _factored_args(spec, dev)
+                coef = g.to(torch.float64).reshape(B).contiguous()
+                hip.call("tedq_sv_lambda_factored", hip.ptr(psi), hip.ptr(lam), B, self.n, len(spec.fbits),
+                         _hptr(bits), _hptr(fac), spec.zmask, hip.ptr(coef), st)
+            elif spec.kind == "probs":
+                k = len(spec.pbits) if spec.pbits else self.n
+                gg = g.to(torch.float32).reshape(B, 2 ** k).contiguous()
+                bits = np.ascontiguousarray(spec.pbits, dtype=np.int32) if spec.pbits else None
+                hip.call("tedq_sv_lambda_probs", hip.ptr(psi), hip.ptr(lam), B, self.n, k, _hptr(bits),
+                         hip.ptr(gg), st)
+            elif spec.kind == "state":
+                gs = g.to(torch.complex64).reshape(B, 2 ** self.n).contiguous()
+                hip.call("tedq_sv_axpy", hip.ptr(lam), hip.ptr(gs), B, self.n, 0.5, st)
+        return lam
+
+    def run_backward(self, psi, lam, amats, mats, grad_dtype=torch.float32):
+        """Adjoint sweep; consumes psi and lam (both are overwritten).  Returns grad [B, P]."""
+        dev = psi.device
+        buf = self._buffers(dev)
+        B = psi.shape[0]
+        st = hip.stream_handle(dev)
+        G = self.plan.n_gslots
+        rows = self.partial_rows()
+        grad = torch.empty((B, max(self.n_params, 1)), dtype=grad_dtype, device=dev)
+        if self.n_params == 0:
+            return grad[:, :0]
+        partials = torch.empty((B, max(G, 1), rows, 8), dtype=torch.float32, device=dev)
+        tl = self._threads_log2(adjoint=True)
+        probe = self.probe
+        for off, mask in zip(self.enc.adj_offsets, reversed(self.enc.tile_masks)):
+            if probe:
+                probe.begin("sv_pass_adj")
+            hip.call("tedq_sv_adjoint_pass", hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r, tl,
+                     buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, st)
+            if probe:
+                probe.end("sv_pass_adj")
+        C = torch.empty((B, max(G, 1), 8), dtype=torch.float64, device=dev)
+        hip.call("tedq_sv_grad_reduce", hip.ptr(partials), B, G, rows, hip.ptr(amats), hip.ptr(buf.pgslot),
+                 self.n_params, hip.ptr(C), hip.ptr(grad), int(grad_dtype == torch.float64), st)
+        return grad
+
+
+def _factored_args(spec, device):  # pylint: disable=unused-argument
+    """Host arrays (the library copies them into the kernel arguments)."""
+    bits = np.ascontiguousarray(spec.fbits, dtype=np.int32)
+    fac = np.ascontiguousarray(spec.ffactors.astype(np.complex64)).view(np.float32).reshape(-1)
+    return bits, fac
+
+
+def _hptr(a):
+    return None if a is None else a.ctypes.data_as(hip.c_vp)

@@ -1,0 +1,117 @@
+"""Circuit -> bra/ket tensor network, one network per measurement.
+
+Restates the index bookkeeping of the reference ``gen_tensor_networks``
+(tedq/tensor_network/tensor_network.py:850-1099) and ``get_symbol``
+(:1109-1127):
+
+* n initial |0> vectors with indices [a_q];
+* every gate in program order gets indices [new..., old...] (output wires
+  first, then the wires it consumes), matching a gate tensor stored as
+  matrix.reshape([2]*2k) with (out..., in...) axes;
+* per measurement:
+    - ``state``: output indices are the current wires, nothing else is added;
+    - ``expval``: the observable tensor(s) are appended like gates, then the
+      adjoint gates in reverse program order, then n closing <0| vectors;
+    - ``probs(qubits)``: the current wires of ``qubits`` become output
+      indices (shared by ket and bra, i.e. a diagonal), then adjoint gates
+      and closing vectors as for ``expval``.
+
+Each tensor also records where its data comes from at execution time, so
+the device executor can assemble the operands for any parameter batch.
+"""
+
+_EINSUM_SYMBOLS_BASE = "abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ"
+
+
+def get_symbol(i):
+    """Index name of integer ``i``: the 52 ASCII letters, then ``chr(i + 140)``."""
+    if i < 52:
+        return _EINSUM_SYMBOLS_BASE[i]
+    return chr(i + 140)
+
+
+class TensorNetwork:
+    """Index structure of one network plus the data source of every tensor.
+
+    ``sources[t]`` is one of ``("zero",)``, ``("gate", instance_id)``,
+    ``("adj", instance_id)``, ``("obs", k)`` (k-th observable factor of the
+    measurement) or ``("obsmat",)`` (the whole observable matrix).
+    """
+
+    def __init__(self, input_indices, output_indices, size_dict, sources, measurement):
+        if len(input_indices) != len(sources):
+            raise ValueError(
+                f"Error! number of input arrays {len(sources)} and input indices {len(input_indices)} do not match!!")
+        used = set()
+        for ix in input_indices:
+            used |= set(ix)
+        used |= set(output_indices)
+        if used ^ set(size_dict):
+            raise ValueError("Error! indices of input do not match with size_dict!!")
+        self.input_indices = input_indices
+        self.output_indices = output_indices
+        self.size_dict = size_dict
+        self.sources = sources
+        self.measurement = measurement
+
+    @property
+    def num_tensors(self):
+        return len(self.input_indices)
+
+
+def _append_op(indices, sources, layer, cur, qubits, source):
+    k = len(qubits)
+    cur += k
+    new = [get_symbol(cur - (k - 1 - i)) for i in range(k)]
+    old = [get_symbol(layer[q]) for q in qubits]
+    indices.append(new + old)
+    sources.append(source)
+    for i, q in enumerate(qubits):
+        layer[q] = cur - (k - 1 - i)
+    return cur
+
+
+def gen_tensor_networks(num_qubits, operators, appliedqubits, measurements):
+    """One ``TensorNetwork`` per measurement (see module docstring)."""
+    from ..measurement import Expectation, Probability, State  # pylint: disable=import-outside-toplevel
+
+    layer = list(range(num_qubits))
+    cur = num_qubits - 1
+    base_ix = [[get_symbol(i)] for i in range(num_qubits)]
+    base_src = [("zero",)] * num_qubits
+    for idx, qbts in appliedqubits.items():
+        cur = _append_op(base_ix, base_src, layer, cur, list(qbts), ("gate", idx))
+
+    networks = []
+    for meas in measurements:
+        ix = [list(x) for x in base_ix]
+        src = list(base_src)
+        lay = list(layer)
+        c = cur
+        out = []
+        if meas.return_type is State:
+            out = [get_symbol(lay[i]) for i in range(num_qubits)]
+        else:
+            if meas.return_type is Expectation:
+                if isinstance(meas.obs, list):
+                    for k, ob in enumerate(meas.obs):
+                        c = _append_op(ix, src, lay, c, [ob.qubits[0]], ("obs", k))
+                else:
+                    c = _append_op(ix, src, lay, c, list(meas.obs.qubits), ("obsmat",))
+            elif meas.return_type is Probability and meas.qubits is not None:
+                out = [get_symbol(lay[q]) for q in meas.qubits]
+            for idx, qbts in reversed(appliedqubits.items()):
+                if len(qbts) > 3:
+                    raise ValueError("Error!! unknown operator with len of applied qubits larger than 3!")
+                c = _append_op(ix, src, lay, c, list(qbts), ("adj", idx))
+            for i in lay:
+                ix.append([get_symbol(i)])
+                src.append(("zero",))
+        size = {}
+        for t in ix:
+            for s in t:
+                size[s] = 2
+        for s in out:
+            size[s] = 2
+        networks.append(TensorNetwork(ix, out, size, src, meas))
+    return networks

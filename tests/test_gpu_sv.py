@@ -1,0 +1,281 @@
+"""GPU parity of the state-vector engine against the complex128 oracle.
+
+Tolerances: amplitudes and expectation values of the complex64 engine are
+compared to the complex128 oracle with max|x - x128| / max(|x128|_inf, 1e-12)
+<= 1e-5 (the stated target) unless a test says otherwise; reference golden
+constants are compared at 5 decimals as the reference's own tests do.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import tedq_amd as qai
+from oracle import sv_oracle as O
+from tedq_amd.backends.pytorch_backend import PyTorchBackend
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda", 0)
+TOL = 1e-5
+
+
+def relerr(x, ref):
+    x, ref = np.asarray(x), np.asarray(ref)
+    return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-12))
+
+
+def _golden(meas):
+    def circuit_def(*params):
+        qai.RX(params[0], qubits=[0])
+        qai.RY(params[1], qubits=[0])
+        return meas()
+    a = torch.tensor([0.54], dtype=torch.float32, device=DEV, requires_grad=True)
+    b = torch.tensor([0.12], dtype=torch.float32, device=DEV, requires_grad=True)
+    return circuit_def, a, b
+
+
+def test_device_info():
+    from tedq_amd.engine import hip
+    info = hip.device_info()
+    print("device", info, flush=True)
+    assert info["wave"] == 64
+    assert info["cu_count"] >= 1
+
+
+def test_golden_forward():
+    # reference test/test_pytorch_backend.py:386-474
+    cd, a, b = _golden(lambda: [qai.expval(qai.PauliZ(qubits=[0])), qai.expval(qai.PauliX(qubits=[1]))])
+    c = qai.Circuit(cd, 2, a, b).compilecircuit(backend="pytorch")
+    r = c(a, b)
+    assert r.dtype == torch.float32 and r.device == a.device
+    assert np.array_equal(np.round(r.detach().cpu().numpy(), 5), np.round(np.array([0.85154057, 0.0], np.float32), 5))
+    cd, a, b = _golden(lambda: qai.probs())
+    r = qai.Circuit(cd, 2, a, b).compilecircuit(backend="pytorch")(a, b)
+    want = np.round(np.array([[0.9257702, 0.], [0.07422972, 0.]], dtype=np.float32), 5)
+    assert np.array_equal(np.round(np.squeeze(r.detach().cpu().numpy()), 5), want)
+    cd, a, b = _golden(lambda: qai.probs(qubits=[1]))
+    r = qai.Circuit(cd, 2, a, b).compilecircuit(backend="pytorch")(a, b)
+    assert np.array_equal(np.round(np.squeeze(r.detach().cpu().numpy()), 5), np.array([1.0, 0.0], np.float32))
+    cd, a, b = _golden(lambda: qai.state())
+    r = qai.Circuit(cd, 2, a, b).compilecircuit(backend="pytorch")(a, b)
+    want = np.round(np.array([[0.9620366 + 0.01599429j, 0], [0.05779156 - 0.26625147j, 0]], np.complex64), 5)
+    assert r.dtype == torch.complex64
+    assert np.array_equal(np.round(np.squeeze(r.detach().cpu().numpy()), 5), want)
+
+
+@pytest.mark.parametrize("method", ["back_prop", "param_shift"])
+def test_golden_backward(method):
+    # reference test/test_pytorch_backend.py:478-586
+    cd, a, b = _golden(lambda: [qai.expval(qai.PauliZ(qubits=[0])), qai.expval(qai.PauliX(qubits=[1]))])
+    c = qai.Circuit(cd, 2, a, b).compilecircuit(backend="pytorch", diff_method=method)
+    r = c(a, b)
+    if method == "param_shift":
+        assert type(r.grad_fn).__name__ == "TorchExecuteBackward"
+    r[0].backward()
+    assert round(float(a.grad), 5) == round(-0.5104387, 5)
+    assert round(float(b.grad), 5) == round(-0.10267819, 5)
+    a2 = torch.tensor([0.54], device=DEV, requires_grad=True)
+    b2 = torch.tensor([0.12], device=DEV, requires_grad=True)
+    c(a2, b2)[1].backward()
+    assert round(float(a2.grad), 5) == 0.0 and round(float(b2.grad), 5) == 0.0
+
+
+def test_init_state_and_cpu_params():
+    for n in (1, 3, 7):
+        c = qai.Circuit(lambda: qai.state(), n).compilecircuit(backend="pytorch")
+        s = c()
+        want = np.zeros([2] * n, np.complex64)
+        want.reshape(-1)[0] = 1
+        assert np.array_equal(np.round(s[0].numpy(), 5), want)
+    # CPU arguments: computed on the GPU, returned on the CPU with gradients
+    a = torch.tensor([0.54], requires_grad=True)
+    b = torch.tensor([0.12], requires_grad=True)
+
+    def cd(*p):
+        qai.RX(p[0], qubits=[0])
+        qai.RY(p[1], qubits=[0])
+        return qai.expval(qai.PauliZ(qubits=[0]))
+    c = qai.Circuit(cd, 1, a, b).compilecircuit(backend="pytorch")
+    r = c(a, b)
+    assert r.device.type == "cpu"
+    r.sum().backward()
+    assert abs(float(a.grad) + math.sin(0.54) * math.cos(0.12)) < 1e-6
+
+
+ALL = ["Hadamard", "PauliX", "PauliY", "PauliZ", "S", "T", "SX", "I", "RX", "RY", "RZ", "Rot", "PhaseShift",
+       "CNOT", "CZ", "CY", "SWAP", "ControlledPhaseShift", "CRX", "CRY", "CRZ", "CSWAP", "Toffoli"]
+
+
+def _random_spec(n, depth, rng):
+    spec = []
+    for _ in range(depth):
+        name = ALL[rng.integers(len(ALL))]
+        k = 3 if name in ("CSWAP", "Toffoli") else 2 if (name.startswith("C") or name == "SWAP") else 1
+        if k > n:
+            continue
+        spec.append((name, [int(q) for q in rng.choice(n, size=k, replace=False)], O.PARAM_COUNT.get(name, 0)))
+    return spec
+
+
+def _build(n, spec, meas_fn, **kw):
+    npar = sum(s[2] for s in spec)
+
+    def cd(*p):
+        it = iter(p)
+        for name, qubits, k in spec:
+            getattr(qai, name)(*[next(it) for _ in range(k)], qubits=qubits)
+        return meas_fn()
+    return qai.Circuit(cd, n, *([0.1] * npar)).compilecircuit(backend="pytorch", **kw), npar
+
+
+def _oracle_gates(spec, vals):
+    it = iter(vals)
+    return [(nm, q, [next(it) for _ in range(k)]) for nm, q, k in spec]
+
+
+@pytest.mark.parametrize("seed,n,tile", [(0, 4, 12), (1, 9, 12), (2, 13, 12), (3, 14, 10), (4, 11, 6),
+                                         (5, 16, 12)])
+def test_random_circuits_forward_and_gradient(seed, n, tile):
+    print(f"case seed={seed} n={n} tile={tile}", flush=True)
+    rng = np.random.default_rng(seed)
+    spec = _random_spec(n, 60, rng)
+    obs = [int(q) for q in rng.choice(n, size=min(n, 3), replace=False)]
+    meas_fn = lambda: [qai.expval(qai.PauliZ(qubits=[q])) for q in obs]  # noqa: E731
+    c, npar = _build(n, spec, meas_fn, tile_bits=tile)
+    vals = rng.uniform(-math.pi, math.pi, npar)
+    theta = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = c(theta)
+    dy = torch.tensor(rng.normal(size=len(obs)), dtype=torch.float32, device=DEV)
+    (out * dy).sum().backward()
+    t64 = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, _oracle_gates(spec, list(t64)), [("expval", ("PauliZ", [q])) for q in obs])
+    (ref * dy.double().cpu()).sum().backward()
+    e_f = relerr(out.detach().cpu().numpy(), ref.detach().numpy())
+    e_g = relerr(theta.grad.cpu().numpy(), t64.grad.numpy())
+    print(f"  forward relerr {e_f:.2e} gradient relerr {e_g:.2e}", flush=True)
+    assert e_f < TOL
+    assert e_g < 1e-4
+
+
+@pytest.mark.parametrize("n", [5, 12])
+def test_state_probs_and_nondiagonal_observables(n):
+    rng = np.random.default_rng(100 + n)
+    spec = _random_spec(n, 40, rng)
+    c_state, npar = _build(n, spec, lambda: qai.state())
+    vals = rng.uniform(-math.pi, math.pi, npar)
+    theta = torch.tensor(vals, dtype=torch.float32, device=DEV)
+    st = c_state(theta)[0].cpu().numpy()
+    ref = O.propagate(n, _oracle_gates(spec, vals)).numpy()
+    assert relerr(st, ref) < TOL
+    c_p, _ = _build(n, spec, lambda: qai.probs(qubits=[n - 1, 0, 2]))
+    p = c_p(theta)[0].cpu().numpy()
+    refp = O.measure(torch.as_tensor(ref), n, [("probs", [n - 1, 0, 2])])[0].numpy()
+    assert p.shape == refp.shape and relerr(p, refp) < TOL
+    obs_sets = [lambda: qai.expval(qai.PauliX(qubits=[1])), lambda: qai.expval(qai.PauliY(qubits=[0])),
+                lambda: qai.expval(qai.Hadamard(qubits=[2])),
+                lambda: qai.expval([qai.PauliX(qubits=[0]), qai.PauliZ(qubits=[1]), qai.PauliY(qubits=[3])])]
+    oracle_obs = [("PauliX", [1]), ("PauliY", [0]), ("Hadamard", [2]),
+                  [("PauliX", [0]), ("PauliZ", [1]), ("PauliY", [3])]]
+    for mf, ob in zip(obs_sets, oracle_obs):
+        c, _ = _build(n, spec, mf)
+        t = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+        r = c(t)
+        r.sum().backward()
+        t64 = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+        rr = O.execute(n, _oracle_gates(spec, list(t64)), [("expval", ob)])
+        rr.sum().backward()
+        assert abs(r.item() - rr.item()) < 1e-5
+        assert relerr(t.grad.cpu().numpy(), t64.grad.numpy()) < 1e-4
+
+
+def test_probs_and_state_gradients_composite_cost():
+    # reference test/test_jax_backend.py:385-428 restated per measurement (torch.stack needs equal shapes)
+    vals = (0.54, 0.12)
+    outs = {}
+    tensors = []
+    for key, mf in [("e", lambda: qai.expval(qai.PauliZ(qubits=[0]))), ("p", lambda: qai.probs()),
+                    ("q", lambda: qai.probs(qubits=[0])), ("s", lambda: qai.state())]:
+        def cd(*p, mf=mf):
+            qai.RX(p[0], qubits=[0])
+            qai.RY(p[1], qubits=[0])
+            return mf()
+        a = torch.tensor([vals[0]], device=DEV, requires_grad=True)
+        b = torch.tensor([vals[1]], device=DEV, requires_grad=True)
+        outs[key] = qai.Circuit(cd, 2, a, b).compilecircuit(backend="pytorch")(a, b)[0]
+        tensors.append((a, b))
+    cost = -torch.tensordot(outs["p"].to(torch.complex64), outs["s"]) + torch.exp(outs["q"].sum()) + 3.3 * outs["e"]
+    torch.abs(cost).backward()
+    ga = sum(float(a.grad) for a, _ in tensors)
+    gb = sum(float(b.grad) for _, b in tensors)
+    assert round(ga, 5) == round(-1.3297417, 5) or abs(ga + 1.3297417) < 2e-5
+    assert abs(gb + 0.30148667) < 2e-5
+
+
+def test_batch_equals_single_and_c1_config():
+    n, L = 8, 4
+    rng = np.random.default_rng(0)
+    theta = rng.uniform(0, 2 * math.pi, size=(6, L, n, 2))
+
+    def cd(*p):
+        it = iter(p)
+        for _ in range(L):
+            for q in range(n):
+                qai.RY(next(it), qubits=[q])
+                qai.RZ(next(it), qubits=[q])
+            for q in range(n - 1):
+                qai.CNOT(qubits=[q, q + 1])
+        return qai.expval(qai.PauliZ(qubits=[0]))
+    c = qai.Circuit(cd, n, *([0.1] * (2 * n * L))).compilecircuit(backend="pytorch")
+    rows = torch.tensor(theta.reshape(6, -1), dtype=torch.float32, device=DEV, requires_grad=True)
+    out = c.execute_batch(rows)
+    out.sum().backward()
+    for i in range(6):
+        t = torch.tensor(theta[i].reshape(-1), dtype=torch.float32, device=DEV, requires_grad=True)
+        single = c(t)
+        single.sum().backward()
+        assert abs(single.item() - out[i, 0].item()) < 1e-6
+        assert torch.allclose(t.grad, rows.grad[i], atol=1e-6)
+        t64 = torch.tensor(theta[i], dtype=torch.float64, requires_grad=True)
+        ref = O.execute(n, O.hea_gates(n, t64)[0], [("expval", ("PauliZ", [0]))])
+        ref.sum().backward()
+        assert relerr(out[i].detach().cpu().numpy(), ref.detach().numpy()) < TOL
+        assert relerr(rows.grad[i].cpu().numpy(), t64.grad.numpy().reshape(-1)) < TOL
+
+
+def test_c2_full_size_properties():
+    """20 qubits, 10 layers, batch 8: fp64 norm, <Z_i> vs oracle for one row, adjoint uncompute."""
+    n, L, B = 20, 10, 8
+    rng = np.random.default_rng(1)
+    theta = rng.uniform(0, 2 * math.pi, size=(B, L, n, 2))
+
+    def cd(*p):
+        it = iter(p)
+        for _ in range(L):
+            for q in range(n):
+                qai.RY(next(it), qubits=[q])
+                qai.RZ(next(it), qubits=[q])
+            for q in range(n - 1):
+                qai.CNOT(qubits=[q, q + 1])
+        return [qai.expval(qai.PauliZ(qubits=[q])) for q in range(n)]
+    c = qai.Circuit(cd, n, *([0.1] * (2 * n * L))).compilecircuit(backend="pytorch")
+    rows = torch.tensor(theta.reshape(B, -1), dtype=torch.float32, device=DEV, requires_grad=True)
+    out = c.execute_batch(rows)
+    ex = c.executor()
+    psi, mats, amats = ex.run_forward(rows.detach(), need_grad=True)
+    norm = (psi.abs() ** 2).double().sum(1)
+    assert torch.allclose(norm, torch.ones_like(norm), atol=1e-5)
+    ref = O.execute(n, O.hea_gates(n, theta[0])[0], [("expval", ("PauliZ", [q])) for q in range(n)])
+    e = relerr(out[0].detach().cpu().numpy(), ref.numpy())
+    print(f"C2 row0 <Z_i> relerr {e:.2e}", flush=True)
+    assert e < TOL
+    out.sum().backward()
+    assert torch.isfinite(rows.grad).all()
+    # the reverse sweep returns psi to |0..0>
+    lam = ex.seed_adjoint(psi, [torch.ones(B, device=DEV) for _ in range(n)])
+    work = psi.clone()
+    ex.run_backward(work, lam, amats, mats)
+    e0 = torch.zeros_like(work)
+    e0[:, 0] = 1
+    assert (work - e0).abs().max().item() < 1e-4

@@ -1,0 +1,151 @@
+"""Fusion plan + device program encoding, checked on the CPU by emulation.
+
+The emulator (tests/sv_emulator.py) executes the int32 programs with the
+kernels' semantics; results must match the oracle's tensordot loop
+(complex128) for states, and the oracle's autograd for adjoint gradients.
+Small tile / register sizes force many passes and stages.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import tedq_amd as qai
+from oracle import sv_oracle as O
+from tedq_amd.engine.lowering import lower_circuit
+from tedq_amd.engine.planner import build_plan
+from tedq_amd.engine.program import encode_plan
+import sv_emulator as EMU
+
+GATES_1Q = ["Hadamard", "PauliX", "PauliY", "PauliZ", "S", "T", "SX", "I"]
+GATES_1P = ["RX", "RY", "RZ", "PhaseShift"]
+GATES_2Q = ["CNOT", "CZ", "CY", "SWAP"]
+GATES_2P = ["CRX", "CRY", "CRZ", "ControlledPhaseShift"]
+GATES_3Q = ["CSWAP", "Toffoli"]
+
+
+def random_spec(n, depth, rng):
+    """List of (name, qubits, n_params) plus inverted flags."""
+    spec = []
+    pools = [GATES_1Q, GATES_1P, GATES_2Q, GATES_2P, GATES_3Q, ["Rot"]]
+    for _ in range(depth):
+        pool = pools[rng.integers(len(pools))]
+        name = pool[rng.integers(len(pool))]
+        k = 3 if pool is GATES_3Q else 2 if pool in (GATES_2Q, GATES_2P) else 1
+        if k > n:
+            continue
+        qubits = [int(q) for q in rng.choice(n, size=k, replace=False)]
+        spec.append((name, qubits, O.PARAM_COUNT.get(name, 0)))
+    return spec
+
+
+def trace(n, spec, values, meas_fn):
+    it = iter(values)
+
+    def circuit_def():
+        for name, qubits, npar in spec:
+            getattr(qai, name)(*[next(it) for _ in range(npar)], qubits=qubits)
+        return meas_fn()
+    return qai.Circuit(circuit_def, n)
+
+
+def oracle_gates(spec, values):
+    it = iter(values)
+    return [(name, qubits, [next(it) for _ in range(npar)]) for name, qubits, npar in spec]
+
+
+@pytest.mark.parametrize("seed,n,k,r", [(0, 5, 3, 2), (1, 6, 4, 2), (2, 7, 5, 3), (3, 6, 6, 4), (4, 8, 5, 2),
+                                        (5, 3, 3, 1), (6, 9, 7, 4)])
+def test_emulated_forward_matches_oracle(seed, n, k, r):
+    rng = np.random.default_rng(seed)
+    spec = random_spec(n, 40, rng)
+    npar = sum(s[2] for s in spec)
+    values = [float(x) for x in rng.uniform(-math.pi, math.pi, npar)]
+    circ = trace(n, spec, values, lambda: qai.expval(qai.PauliZ(qubits=[0])))
+    cc = circ.compilecircuit(backend="pytorch")
+    nq, units, P, pool = lower_circuit(cc)
+    assert P == npar
+    plan = build_plan(nq, units, P, pool, k_max=k, r=r)
+    enc = encode_plan(plan)
+    psi, _, _ = EMU.forward(enc, plan, np.asarray(values))
+    ref = O.propagate(n, oracle_gates(spec, values)).numpy().reshape(-1)
+    assert np.allclose(psi, ref, atol=1e-10)
+    # every tile set has k distinct bits, ascending
+    for p in plan.passes:
+        assert len(p.tile_bits) == min(n, k)
+        assert p.tile_bits == sorted(set(p.tile_bits))
+
+
+@pytest.mark.parametrize("seed,n,k,r", [(10, 5, 3, 2), (11, 6, 4, 3), (12, 7, 7, 4), (13, 8, 5, 2)])
+def test_emulated_adjoint_gradient_matches_autograd(seed, n, k, r):
+    rng = np.random.default_rng(seed)
+    spec = random_spec(n, 30, rng)
+    spec.append(("Rot", [0], 3))
+    npar = sum(s[2] for s in spec)
+    values = [float(x) for x in rng.uniform(-math.pi, math.pi, npar)]
+    obs_q = [int(q) for q in rng.choice(n, size=2, replace=False)]
+    circ = trace(n, spec, values, lambda: [qai.expval(qai.PauliZ(qubits=[obs_q[0]])),
+                                           qai.expval(qai.PauliZ(qubits=[obs_q[1]]))])
+    cc = circ.compilecircuit(backend="pytorch")
+    nq, units, P, pool = lower_circuit(cc)
+    plan = build_plan(nq, units, P, pool, k_max=k, r=r)
+    enc = encode_plan(plan)
+    row = np.asarray(values)
+    psi, mats, amats = EMU.forward(enc, plan, row)
+    dy = rng.normal(size=2)
+    z = np.zeros(2 ** n)
+    for q, d in zip(obs_q, dy):
+        bit = n - 1 - q
+        z += d * (1 - 2 * ((np.arange(2 ** n) >> bit) & 1))
+    lam = z * psi
+    grad, psi0 = EMU.adjoint_grad(enc, plan, psi, lam, mats, amats, P)
+    # the reverse sweep must uncompute psi back to |0..0>
+    e0 = np.zeros(2 ** n)
+    e0[0] = 1
+    assert np.allclose(psi0, e0, atol=1e-10)
+    t = torch.tensor(values, dtype=torch.float64, requires_grad=True)
+    out = O.execute(n, oracle_gates(spec, list(t)), [("expval", ("PauliZ", [q])) for q in obs_q])
+    (out * torch.tensor(dy)).sum().backward()
+    assert np.allclose(grad, t.grad.numpy(), atol=1e-9)
+
+
+def test_trainable_subset_and_order():
+    """Non-trainable parameters become constants; trainable_params order maps call arguments."""
+    def circuit_def(*p):
+        qai.Rot(p[0], p[1], p[2], qubits=[0], trainable_params=[2, 0])
+        qai.RZ(p[3], qubits=[1], trainable_params=[])
+        qai.CRY(p[4], qubits=[1, 0])
+        return qai.expval(qai.PauliZ(qubits=[0]))
+    vals = [0.3, -1.1, 0.7, 0.25, 1.9]
+    cc = qai.Circuit(circuit_def, 2, *vals).compilecircuit(backend="pytorch")
+    n, units, P, pool = lower_circuit(cc)
+    assert P == 3
+    plan = build_plan(n, units, P, pool, k_max=2, r=1)
+    enc = encode_plan(plan)
+    row = np.array([0.7, 0.3, 1.9])      # column 0 -> Rot omega (pos 2), column 1 -> Rot phi (pos 0)
+    psi, _, _ = EMU.forward(enc, plan, row)
+    ref = O.propagate(2, [("Rot", [0], [0.3, -1.1, 0.7]), ("RZ", [1], [0.25]), ("CRY", [1, 0], [1.9])])
+    assert np.allclose(psi, ref.numpy().reshape(-1), atol=1e-12)
+
+
+def test_hea_plan_two_passes_per_layer():
+    """The 20-qubit hardware-efficient ansatz folds into about two passes per layer."""
+    n, L = 20, 10
+
+    def circuit_def(*p):
+        it = iter(p)
+        for _ in range(L):
+            for q in range(n):
+                qai.RY(next(it), qubits=[q])
+                qai.RZ(next(it), qubits=[q])
+            for q in range(n - 1):
+                qai.CNOT(qubits=[q, q + 1])
+        return [qai.expval(qai.PauliZ(qubits=[q])) for q in range(n)]
+    cc = qai.Circuit(circuit_def, n, *([0.1] * (2 * n * L))).compilecircuit(backend="pytorch")
+    nq, units, P, pool = lower_circuit(cc)
+    plan = build_plan(nq, units, P, pool, k_max=12, r=4)
+    assert len(plan.passes) <= 2 * L + 2
+    assert all(p.tile_bits[:3] == [0, 1, 2] for p in plan.passes)   # coalesced 64-byte runs
+    assert plan.n_gslots == n * L       # RY.RZ fused per qubit and layer
