@@ -154,6 +154,70 @@ def cpu_baseline(n, layers, theta, budget_s=15.0):
             "cpu_model": model}
 
 
+def hea_z0_circuit(n, layers):
+    import tedq_amd as qai  # pylint: disable=import-outside-toplevel
+
+    def circuit_def(*p):
+        it = iter(p)
+        for _ in range(layers):
+            for q in range(n):
+                qai.RY(next(it), qubits=[q])
+                qai.RZ(next(it), qubits=[q])
+            for q in range(n - 1):
+                qai.CNOT(qubits=[q, q + 1])
+        return qai.expval(qai.PauliZ(qubits=[0]))
+    return qai.Circuit(circuit_def, n, *([0.1] * (2 * n * layers)))
+
+
+def serving_stream(n_req, seed=4, rate=None):
+    """Requests of the mixed stream: n ~ U{6..24}, layers ~ U{2..20}, params ~ U[0, 2pi)."""
+    from tedq_amd.serving.engine import Request  # pylint: disable=import-outside-toplevel
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(6, 25, size=n_req)
+    ls = rng.integers(2, 21, size=n_req)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_req)) if rate else np.zeros(n_req)
+    return [Request(i, (int(n), int(L)), rng.uniform(0, 2 * np.pi, 2 * int(n) * int(L)).astype(np.float32),
+                    float(t[i])) for i, (n, L) in enumerate(zip(ns, ls))]
+
+
+def serving_leg(n_req, warm):
+    """Closed-loop capacity, Poisson stream at 0.7x capacity (p50/p99), and the unbatched path."""
+    from tedq_amd.serving.engine import ServingEngine, latency_stats  # pylint: disable=import-outside-toplevel
+    compiled = {}
+
+    def factory(key):
+        if key not in compiled:
+            compiled[key] = hea_z0_circuit(*key).compilecircuit(backend="pytorch")
+        return compiled[key]
+
+    eng = ServingEngine(factory, hbm_budget=64 << 30, max_batch=64, n_streams=4, max_inflight=8)
+    closed = serving_stream(n_req)
+    keys = sorted({r.key for r in closed} | {r.key for r in serving_stream(n_req, seed=5, rate=1.0)})
+    eng.prepare(keys, buckets=(1, 2, 4))
+    eng.run(serving_stream(warm, seed=99))
+    eng.completed.clear()
+    eng.batch_sizes.clear()
+    eng.run(closed)
+    cap = latency_stats(closed)
+    rate = 0.7 * cap["evals_per_s"]
+    eng.completed.clear()
+    eng.batch_sizes.clear()
+    stream = serving_stream(n_req, seed=5, rate=rate)
+    eng.run(stream)
+    st = latency_stats(stream)
+    mean_batch = float(np.mean(eng.batch_sizes))
+    single = ServingEngine(factory, hbm_budget=64 << 30, max_batch=1, n_streams=1, max_inflight=1,
+                           graph_max_qubits=0)
+    sub = serving_stream(min(n_req, 400), seed=6)
+    single.run(sub)
+    un = latency_stats(sub)
+    return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "evals_per_s": st["evals_per_s"], "arrival_rate": rate, "requests": st["requests"],
+            "capacity_evals_per_s": cap["evals_per_s"], "mean_batch": mean_batch,
+            "unbatched_evals_per_s": un["evals_per_s"], "graphs_captured": len(eng._graphs),
+            "stream": "Poisson at 0.7x closed-loop capacity; n~U{6..24}, layers~U{2..20}; <Z_0> + full gradient"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,7 +230,20 @@ def main():
     ap.add_argument("--reg-bits", type=int, default=4)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (counter calibration for traffic)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        src = torch.ones(2 ** 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine is measured on one GPU; multi-GPU runs are not part of this "
                                    "build (batches shard across GPUs without communication)", "n_gpus": args.gpus}))
@@ -181,6 +258,13 @@ def main():
                                                  reg_bits=args.reg_bits)
     ex = compiled.executor()
     rows = torch.tensor(theta, dtype=torch.float32, device=dev)
+
+    tuning = None
+    if not args.no_autotune:
+        from tedq_amd.engine.autotune import autotune  # pylint: disable=import-outside-toplevel
+        res = autotune(ex, rows)
+        tuning = {"best": list(res["best"]), "fwd_ms": {f"{k}/{r}": v for (k, r), v in res["fwd"].items()},
+                  "bwd_ms": {f"{k}/{r}": v for (k, r), v in res["bwd"].items()}}
 
     def step():
         r = rows.detach().requires_grad_(True)
@@ -220,13 +304,17 @@ def main():
         "vs_baseline": None, "dtype": "c64", "data": "synthetic",
         "config": {"workload": f"hardware-efficient ansatz {n}q {L} layers, batch {B}, <Z_i> for all qubits, "
                                "fwd + adjoint grad", "qubits": n, "layers": L, "batch": B,
-                   "params_per_circuit": 2 * n * L, "tile_bits": ex.plan.k, "reg_bits": ex.plan.r,
-                   "passes": ex.n_passes},
+                   "params_per_circuit": 2 * n * L, "tile_bits": ex.fwd.plan.k, "reg_bits": ex.fwd.plan.r,
+                   "tile_bits_bwd": ex.bwd.plan.k, "reg_bits_bwd": ex.bwd.plan.r,
+                   "passes": ex.n_passes, "passes_bwd": len(ex.bwd.plan.passes)},
+        "autotune": tuning,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": kd["bytes_per_launch"], "mean_launch_ms": kd["mean_ms"]},
         "kernels": kernels,
     }
+    if not args.skip_serving:
+        line["latency"] = serving_leg(args.serve_requests, warm=200)
     if not args.skip_plain:
         ps, cnt = time_plain_gpu(n, L, theta[:16], dev)
         line["plain_gpu"] = {"value": ps, "unit": "evals/s", "sample": f"{cnt} single-row calls",

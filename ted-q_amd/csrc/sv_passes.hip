@@ -17,14 +17,14 @@
 //   [4..36)   physical position of tile bit i (ascending)
 //   [36..100) 64-bit physical offset of register-row m in the load layout
 //   [100 + 48 s ...] stage s: [0] nops [1] op word offset [2] non-register
-//             local-bit mask [3] unused [4..36) local offset of register m
+//             local-bit mask [3] register mask [4..36) swizzled local offset of register m
 //   ops: 8 words {type | dagger<<8, b0, b1, cmask, mat_off, gslot, 0, 0}
 #include "tedq_common.h"
 
 namespace tedq {
 
 enum OpCode : int {
-  kOpU1 = 1, kOpD1 = 2, kOpCU1 = 3, kOpCX = 4, kOpCZ = 5, kOpU2 = 6,
+  kOpU1 = 1, kOpD1 = 2, kOpCU1 = 3, kOpCX = 4, kOpCZ = 5,  // 6: unused
   kOpGrad1 = 7, kOpCGrad1 = 8, kOpCD1 = 9,
 };
 
@@ -34,8 +34,6 @@ constexpr int kHdrStages = 100;
 constexpr int kStageWords = 48;
 
 struct PassArgs {
-  const int* prog;
-  const float2* mats;
   int64_t mat_stride;
   int64_t batch;
   int64_t total_tiles;      // batch * 2^(n-k)
@@ -44,16 +42,87 @@ struct PassArgs {
   int k;
   int tT;                   // log2(threads per tile)
   int gT;                   // log2(tiles per workgroup)
-  float* partials;          // adjoint only: [batch][n_gslots][np][8]
-  int64_t np;               // partial rows per gradient slot per circuit
+  int64_t np;               // adjoint: partials are [batch][n_gslots][np][8]               // partial rows per gradient slot per circuit
   int n_gslots;
 };
 
 // ---------------------------------------------------------------------------
-// register-level gate kernels; B, B0, B1 are register-bit indices
+// register-level gate kernels.  B is the target register bit and CM a control
+// mask over register bits, both compile-time, so every pair selection below
+// folds away: an op costs only its arithmetic (no per-pair scalar tests).
+// Ops with several control bits (rare: Toffoli) use the *_rt runtime forms.
 // ---------------------------------------------------------------------------
+template <int R, int B, int CM>
+struct U1Op {  // general 2x2 on bit B where all CM bits are 1
+  static __device__ __forceinline__ void run(float2 (&v)[R], float2 m0, float2 m1, float2 m2, float2 m3) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & (1 << B)) || (i & CM) != CM) continue;
+      const int j = i | (1 << B);
+      const float2 x = v[i], y = v[j];
+      v[i] = cmul2(m0, x, m1, y);
+      v[j] = cmul2(m2, x, m3, y);
+    }
+  }
+};
+
+template <int R, int B, int CM>
+struct DiagOp {  // diag(d0, d1) on bit B where all CM bits are 1
+  static __device__ __forceinline__ void run(float2 (&v)[R], float2 d0, float2 d1) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & CM) != CM) continue;
+      v[i] = cmul((i >> B) & 1 ? d1 : d0, v[i]);
+    }
+  }
+};
+
+template <int R, int B, int CM>
+struct CXOp {  // X on bit B where all CM bits are 1
+  static __device__ __forceinline__ void run(float2 (&v)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & (1 << B)) || (i & CM) != CM) continue;
+      const int j = i | (1 << B);
+      const float2 t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+    }
+  }
+};
+
+template <int R, int B, int CM>
+struct CZOp {  // -1 where bit B and all CM bits are 1
+  static __device__ __forceinline__ void run(float2 (&v)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if ((i & (1 << B)) && (i & CM) == CM) v[i] = cneg(v[i]);
+  }
+};
+
+// bilinears c_ab = sum conj(l_a) p_b over the pairs of bit B where all CM bits are 1
+template <int R, int B, int CM>
+struct GradOp {
+  static __device__ __forceinline__ void run(const float2 (&p)[R], const float2 (&l)[R], float (&acc)[8]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & (1 << B)) || (i & CM) != CM) continue;
+      const int j = i | (1 << B);
+      acc[0] = fmaf(l[i].x, p[i].x, fmaf(l[i].y, p[i].y, acc[0]));
+      acc[1] = fmaf(l[i].x, p[i].y, fmaf(-l[i].y, p[i].x, acc[1]));
+      acc[2] = fmaf(l[i].x, p[j].x, fmaf(l[i].y, p[j].y, acc[2]));
+      acc[3] = fmaf(l[i].x, p[j].y, fmaf(-l[i].y, p[j].x, acc[3]));
+      acc[4] = fmaf(l[j].x, p[i].x, fmaf(l[j].y, p[i].y, acc[4]));
+      acc[5] = fmaf(l[j].x, p[i].y, fmaf(-l[j].y, p[i].x, acc[5]));
+      acc[6] = fmaf(l[j].x, p[j].x, fmaf(l[j].y, p[j].y, acc[6]));
+      acc[7] = fmaf(l[j].x, p[j].y, fmaf(-l[j].y, p[j].x, acc[7]));
+    }
+  }
+};
+
+// runtime-mask forms for multi-control ops
 template <int R, int B>
-__device__ __forceinline__ void u1_bit(float2 (&v)[R], float2 m0, float2 m1, float2 m2, float2 m3, int cmask) {
+__device__ __forceinline__ void u1_rt(float2 (&v)[R], float2 m0, float2 m1, float2 m2, float2 m3, int cmask) {
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     if (i & (1 << B)) continue;
@@ -64,9 +133,8 @@ __device__ __forceinline__ void u1_bit(float2 (&v)[R], float2 m0, float2 m1, flo
     v[j] = cmul2(m2, x, m3, y);
   }
 }
-
 template <int R, int B>
-__device__ __forceinline__ void cx_bit(float2 (&v)[R], int cmask) {
+__device__ __forceinline__ void cx_rt(float2 (&v)[R], int cmask) {
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     if (i & (1 << B)) continue;
@@ -77,36 +145,8 @@ __device__ __forceinline__ void cx_bit(float2 (&v)[R], int cmask) {
     v[j] = t;
   }
 }
-
-template <int R, int B0, int B1>
-__device__ __forceinline__ void u2_bits(float2 (&v)[R], const float2* m, bool dag) {
-#pragma unroll
-  for (int i = 0; i < R; ++i) {
-    if (i & ((1 << B0) | (1 << B1))) continue;
-    const int i0 = i, i1 = i | (1 << B1), i2 = i | (1 << B0), i3 = i | (1 << B0) | (1 << B1);
-    const float2 x0 = v[i0], x1 = v[i1], x2 = v[i2], x3 = v[i3];
-    float2 y[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      // row r of M (or of M^dagger: conj of column r)
-      const float2 a0 = dag ? cconj(m[0 * 4 + r]) : m[r * 4 + 0];
-      const float2 a1 = dag ? cconj(m[1 * 4 + r]) : m[r * 4 + 1];
-      const float2 a2 = dag ? cconj(m[2 * 4 + r]) : m[r * 4 + 2];
-      const float2 a3 = dag ? cconj(m[3 * 4 + r]) : m[r * 4 + 3];
-      const float2 s0 = cmul2(a0, x0, a1, x1);
-      const float2 s1 = cmul2(a2, x2, a3, x3);
-      y[r] = make_float2(s0.x + s1.x, s0.y + s1.y);
-    }
-    v[i0] = y[0];
-    v[i1] = y[1];
-    v[i2] = y[2];
-    v[i3] = y[3];
-  }
-}
-
-// bilinears c_ab = sum conj(l_a) p_b over the pairs of register bit B
 template <int R, int B>
-__device__ __forceinline__ void grad_bit(const float2 (&p)[R], const float2 (&l)[R], int cmask, float (&acc)[8]) {
+__device__ __forceinline__ void grad_rt(const float2 (&p)[R], const float2 (&l)[R], int cmask, float (&acc)[8]) {
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     if (i & (1 << B)) continue;
@@ -122,6 +162,49 @@ __device__ __forceinline__ void grad_bit(const float2 (&p)[R], const float2 (&l)
     acc[7] = fmaf(l[j].x, p[j].y, fmaf(-l[j].y, p[j].x, acc[7]));
   }
 }
+template <int R>
+__device__ __forceinline__ void diag_rt(float2 (&v)[R], int b, int cmask, float2 d0, float2 d1) {
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    if ((i & cmask) != cmask) continue;
+    v[i] = cmul(((i >> b) & 1) ? d1 : d0, v[i]);
+  }
+}
+template <int R>
+__device__ __forceinline__ void cz_rt(float2 (&v)[R], int mask) {
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+    if ((i & mask) == mask) v[i] = cneg(v[i]);
+}
+
+// Dispatch a runtime (target bit, control mask) to Op<R, B, CM> when the mask
+// has at most one bit; returns false otherwise.
+template <int R, template <int, int, int> class Op, int B, typename... A>
+__device__ __forceinline__ bool by_ctrl(int cmask, A&... a) {
+  switch (cmask) {
+    case 0: Op<R, B, 0>::run(a...); return true;
+    case 1: if constexpr (B != 0 && R > 1) { Op<R, B, 1>::run(a...); return true; } break;
+    case 2: if constexpr (B != 1 && R > 2) { Op<R, B, 2>::run(a...); return true; } break;
+    case 4: if constexpr (B != 2 && R > 4) { Op<R, B, 4>::run(a...); return true; } break;
+    case 8: if constexpr (B != 3 && R > 8) { Op<R, B, 8>::run(a...); return true; } break;
+    case 16: if constexpr (B != 4 && R > 16) { Op<R, B, 16>::run(a...); return true; } break;
+    default: break;
+  }
+  return false;
+}
+
+template <int R, template <int, int, int> class Op, typename... A>
+__device__ __forceinline__ bool by_bit(int b, int cmask, A&... a) {
+  switch (b) {
+    case 0: if constexpr (R > 1) return by_ctrl<R, Op, 0>(cmask, a...); break;
+    case 1: if constexpr (R > 2) return by_ctrl<R, Op, 1>(cmask, a...); break;
+    case 2: if constexpr (R > 4) return by_ctrl<R, Op, 2>(cmask, a...); break;
+    case 3: if constexpr (R > 8) return by_ctrl<R, Op, 3>(cmask, a...); break;
+    case 4: if constexpr (R > 16) return by_ctrl<R, Op, 4>(cmask, a...); break;
+    default: break;
+  }
+  return false;
+}
 
 #define TEDQ_BIT_SWITCH(b, CALL)                        \
   switch (b) {                                          \
@@ -135,14 +218,16 @@ __device__ __forceinline__ void grad_bit(const float2 (&p)[R], const float2 (&l)
 
 template <int R>
 __device__ __forceinline__ void op_u1(float2 (&v)[R], int b, int cmask, float2 m0, float2 m1, float2 m2, float2 m3) {
-#define CALL(B) u1_bit<R, B>(v, m0, m1, m2, m3, cmask)
+  if (by_bit<R, U1Op>(b, cmask, v, m0, m1, m2, m3)) return;
+#define CALL(B) u1_rt<R, B>(v, m0, m1, m2, m3, cmask)
   TEDQ_BIT_SWITCH(b, CALL)
 #undef CALL
 }
 
 template <int R>
 __device__ __forceinline__ void op_cx(float2 (&v)[R], int b, int cmask) {
-#define CALL(B) cx_bit<R, B>(v, cmask)
+  if (by_bit<R, CXOp>(b, cmask, v)) return;
+#define CALL(B) cx_rt<R, B>(v, cmask)
   TEDQ_BIT_SWITCH(b, CALL)
 #undef CALL
 }
@@ -150,39 +235,23 @@ __device__ __forceinline__ void op_cx(float2 (&v)[R], int b, int cmask) {
 template <int R>
 __device__ __forceinline__ void op_grad(const float2 (&p)[R], const float2 (&l)[R], int b, int cmask,
                                         float (&acc)[8]) {
-#define CALL(B) grad_bit<R, B>(p, l, cmask, acc)
+  if (by_bit<R, GradOp>(b, cmask, p, l, acc)) return;
+#define CALL(B) grad_rt<R, B>(p, l, cmask, acc)
   TEDQ_BIT_SWITCH(b, CALL)
-#undef CALL
-}
-
-template <int R, int B0>
-__device__ __forceinline__ void op_u2_b1(float2 (&v)[R], int b1, const float2* m, bool dag) {
-#define CALL(B1) if constexpr (B1 != B0) u2_bits<R, B0, B1>(v, m, dag)
-  TEDQ_BIT_SWITCH(b1, CALL)
-#undef CALL
-}
-
-template <int R>
-__device__ __forceinline__ void op_u2(float2 (&v)[R], int b0, int b1, const float2* m, bool dag) {
-#define CALL(B0) op_u2_b1<R, B0>(v, b1, m, dag)
-  TEDQ_BIT_SWITCH(b0, CALL)
 #undef CALL
 }
 
 template <int R>
 __device__ __forceinline__ void op_diag(float2 (&v)[R], int b, int cmask, float2 d0, float2 d1) {
-#pragma unroll
-  for (int i = 0; i < R; ++i) {
-    if ((i & cmask) != cmask) continue;
-    v[i] = cmul(((i >> b) & 1) ? d1 : d0, v[i]);
-  }
+  if (by_bit<R, DiagOp>(b, cmask, v, d0, d1)) return;
+  diag_rt<R>(v, b, cmask, d0, d1);
 }
 
 template <int R>
 __device__ __forceinline__ void op_cz(float2 (&v)[R], int mask) {
-#pragma unroll
-  for (int i = 0; i < R; ++i)
-    if ((i & mask) == mask) v[i] = cneg(v[i]);
+  const int top = 31 - __builtin_clz(mask | 1);
+  if (by_bit<R, CZOp>(top, mask & ~(1 << top), v)) return;
+  cz_rt<R>(v, mask);
 }
 
 // Apply one unitary op (optionally daggered) to one register set, or to two
@@ -227,10 +296,6 @@ __device__ __forceinline__ void apply_op(float2 (&v)[R], float2 (&w)[R], const i
       op_cz<R>(v, op[3]);
       if (TWO) op_cz<R>(w, op[3]);
       break;
-    case kOpU2:
-      op_u2<R>(v, op[1], op[2], mb + op[4], dag);
-      if (TWO) op_u2<R>(w, op[1], op[2], mb + op[4], dag);
-      break;
     default: break;
   }
 }
@@ -245,45 +310,81 @@ struct TileCtx {
   bool valid;
 };
 
+typedef int tedq_v2i __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  // raw buffer, stride 0, num_records in bytes; 0x00020000 is the gfx9 data-format word
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(float2 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tedq_v2i, v), r, voff, soff, 0);
+}
+
+// Tiles are moved with buffer instructions: the resource (tile base, uniform
+// per workgroup) lives in SGPRs, each lane adds a 32-bit offset and each
+// register row a scalar offset, so no 64-bit address is held per element.
+// Out-of-range buffer loads return 0 and stores are dropped, which covers the
+// partial last workgroup of whole-circuit tiles.  States with more than 2^28
+// amplitudes per circuit use plain global addressing.
+//
+// The swizzle is XOR-linear and a thread's LDS index is (thread part) |
+// (register part) with disjoint bits, so swz(base | off) = swz(base) ^
+// swz(off): register offsets in the stage header are stored pre-swizzled and
+// each element costs one XOR.
 template <int R, bool MULTI>
-__device__ __forceinline__ void load_tiles(const float2* __restrict__ src, float2* lds, const PassArgs& a,
-                                           int tid, int T, const TileCtx& c) {
+__device__ __forceinline__ void load_tiles(const float2* __restrict__ src, float2* lds, const int* __restrict__ prog,
+                                           const PassArgs& a, int tid, int T, const TileCtx& c) {
+  const int sw = swz(tid);
   if (MULTI) {
     const int64_t wg_base = (int64_t)blockIdx.x * T * R;
-    const int64_t limit = a.batch << a.n;
+    const int64_t left = (a.batch << a.n) - wg_base;
+    const int64_t cnt = left < (int64_t)T * R ? left : (int64_t)T * R;
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(src + wg_base, (uint32_t)(cnt * 8));
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int e = tid + m * T;
-      const int64_t gi = wg_base + e;
-      lds[swz(e)] = gi < limit ? src[gi] : make_float2(0.f, 0.f);
-    }
+    for (int m = 0; m < R; ++m) lds[sw ^ swz(m * T)] = bload(r, (uint32_t)tid * 8u, (uint32_t)(m * T) * 8u);
+  } else if (a.n <= 28) {
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(src + c.gbase, (uint32_t)(8u << a.n));
+    const int64_t* hi = reinterpret_cast<const int64_t*>(prog + kHdrHiOff);
+    const uint32_t vo = (uint32_t)c.lo * 8u;
+#pragma unroll
+    for (int m = 0; m < R; ++m) lds[sw ^ swz(m * T)] = bload(r, vo, (uint32_t)hi[m] * 8u);
   } else {
-    const int64_t* hi = reinterpret_cast<const int64_t*>(a.prog + kHdrHiOff);
+    const int64_t* hi = reinterpret_cast<const int64_t*>(prog + kHdrHiOff);
+    const float2* p = src + c.gbase + c.lo;
 #pragma unroll
-    for (int m = 0; m < R; ++m) lds[swz(tid + m * T)] = src[c.gbase + c.lo + hi[m]];
+    for (int m = 0; m < R; ++m) lds[sw ^ swz(m * T)] = p[hi[m]];
   }
 }
 
 template <int R, bool MULTI>
-__device__ __forceinline__ void store_tiles(float2* __restrict__ dst, const float2* lds, const PassArgs& a,
-                                            int tid, int T, const TileCtx& c) {
+__device__ __forceinline__ void store_tiles(float2* __restrict__ dst, const float2* lds, const int* __restrict__ prog,
+                                            const PassArgs& a, int tid, int T, const TileCtx& c) {
+  const int sw = swz(tid);
   if (MULTI) {
     const int64_t wg_base = (int64_t)blockIdx.x * T * R;
-    const int64_t limit = a.batch << a.n;
+    const int64_t left = (a.batch << a.n) - wg_base;
+    const int64_t cnt = left < (int64_t)T * R ? left : (int64_t)T * R;
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(dst + wg_base, (uint32_t)(cnt * 8));
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int e = tid + m * T;
-      const int64_t gi = wg_base + e;
-      if (gi < limit) dst[gi] = lds[swz(e)];
-    }
+    for (int m = 0; m < R; ++m) bstore(lds[sw ^ swz(m * T)], r, (uint32_t)tid * 8u, (uint32_t)(m * T) * 8u);
+  } else if (a.n <= 28) {
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(dst + c.gbase, (uint32_t)(8u << a.n));
+    const int64_t* hi = reinterpret_cast<const int64_t*>(prog + kHdrHiOff);
+    const uint32_t vo = (uint32_t)c.lo * 8u;
+#pragma unroll
+    for (int m = 0; m < R; ++m) bstore(lds[sw ^ swz(m * T)], r, vo, (uint32_t)hi[m] * 8u);
   } else {
-    const int64_t* hi = reinterpret_cast<const int64_t*>(a.prog + kHdrHiOff);
+    const int64_t* hi = reinterpret_cast<const int64_t*>(prog + kHdrHiOff);
+    float2* p = dst + c.gbase + c.lo;
 #pragma unroll
-    for (int m = 0; m < R; ++m) dst[c.gbase + c.lo + hi[m]] = lds[swz(tid + m * T)];
+    for (int m = 0; m < R; ++m) p[hi[m]] = lds[sw ^ swz(m * T)];
   }
 }
 
-__device__ __forceinline__ TileCtx make_ctx(const PassArgs& a, int tid, bool multi) {
+__device__ __forceinline__ TileCtx make_ctx(const PassArgs& a, const int* __restrict__ prog, int tid, bool multi) {
   TileCtx c;
   if (multi) {
     const int64_t tile = ((int64_t)blockIdx.x << a.gT) + (tid >> a.tT);
@@ -300,12 +401,13 @@ __device__ __forceinline__ TileCtx make_ctx(const PassArgs& a, int tid, bool mul
     c.gbase = (c.circuit << a.n) + (int64_t)pdep64(within, a.rest_mask);
     uint64_t lo = 0;
     for (int i = 0; i < a.tT; ++i)
-      if ((tid >> i) & 1) lo |= 1ull << a.prog[kHdrTileBits + i];
+      if ((tid >> i) & 1) lo |= 1ull << prog[kHdrTileBits + i];
     c.lo = lo;
   }
   return c;
 }
 
+// swizzled LDS index of this thread's register 0 in a stage
 __device__ __forceinline__ int stage_base(int tid, int tT, int k, int nonreg) {
   const int g = tid >> tT;
   int tl = tid & ((1 << tT) - 1);
@@ -317,36 +419,66 @@ __device__ __forceinline__ int stage_base(int tid, int tT, int k, int nonreg) {
     tl >>= 1;
     mask &= mask - 1;
   }
-  return (g << k) + base;
+  return swz((g << k) + base);
+}
+
+// Sum the 8 bilinears over the 64 lanes of a wave by recursive halving:
+// after the xor-32/16/8 steps lane L holds value (L >> 3) & 7 summed over lane
+// bits 3..5, then three butterfly steps finish bits 0..2 (10 shuffles instead
+// of 48).  Returns the total for value (lane >> 3) & 7, valid on lanes with
+// (lane & 7) == 0.
+__device__ __forceinline__ float reduce8_wave(float (&v)[8], int lane) {
+  float h4[4], h2[2];
+  const bool up5 = lane & 32;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float send = up5 ? v[j] : v[4 + j];
+    const float keep = up5 ? v[4 + j] : v[j];
+    h4[j] = keep + __shfl_xor(send, 32, 64);
+  }
+  const bool up4 = lane & 16;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const float send = up4 ? h4[j] : h4[2 + j];
+    const float keep = up4 ? h4[2 + j] : h4[j];
+    h2[j] = keep + __shfl_xor(send, 16, 64);
+  }
+  const bool up3 = lane & 8;
+  float x = (up3 ? h2[1] : h2[0]) + __shfl_xor(up3 ? h2[0] : h2[1], 8, 64);
+  x += __shfl_xor(x, 4, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 1, 64);
+  return x;
 }
 
 // ---------------------------------------------------------------------------
 // forward pass
 // ---------------------------------------------------------------------------
 template <int R, bool MULTI>
-__global__ void __launch_bounds__(256) k_pass_fwd(float2* __restrict__ state, PassArgs a) {
+__global__ void __launch_bounds__(512) k_pass_fwd(float2* __restrict__ state, const int* __restrict__ prog,
+                                                  const float2* __restrict__ mats, PassArgs a) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x;
   const int T = blockDim.x;
-  const TileCtx c = make_ctx(a, tid, MULTI);
-  const float2* mb = a.mats + (c.valid ? c.circuit : 0) * a.mat_stride;
-  load_tiles<R, MULTI>(state, lds, a, tid, T, c);
+  const TileCtx c = make_ctx(a, prog, tid, MULTI);
+  const float2* mb = mats + (c.valid ? c.circuit : 0) * a.mat_stride;
+  load_tiles<R, MULTI>(state, lds, prog, a, tid, T, c);
   __syncthreads();
-  const int nst = a.prog[2];
+  const int nst = prog[2];
   for (int s = 0; s < nst; ++s) {
-    const int* st = a.prog + kHdrStages + kStageWords * s;
-    const int base = stage_base(tid, a.tT, a.k, st[2]);
+    const int* st = prog + kHdrStages + kStageWords * s;
+    const int sb = stage_base(tid, a.tT, a.k, st[2]);
     float2 v[R];
 #pragma unroll
-    for (int m = 0; m < R; ++m) v[m] = lds[swz(base + st[4 + m])];
+    for (int m = 0; m < R; ++m) v[m] = lds[sb ^ st[4 + m]];
     const int nops = st[0];
-    const int* op = a.prog + st[1];
+    const int* op = prog + st[1];
     for (int o = 0; o < nops; ++o, op += 8) apply_op<R, false>(v, v, op, mb);
 #pragma unroll
-    for (int m = 0; m < R; ++m) lds[swz(base + st[4 + m])] = v[m];
+    for (int m = 0; m < R; ++m) lds[sb ^ st[4 + m]] = v[m];
     __syncthreads();
   }
-  store_tiles<R, MULTI>(state, lds, a, tid, T, c);
+  store_tiles<R, MULTI>(state, lds, prog, a, tid, T, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -354,16 +486,19 @@ __global__ void __launch_bounds__(256) k_pass_fwd(float2* __restrict__ state, Pa
 // lambda; gradient ops reduce their 8 bilinears over the threads of a tile.
 // ---------------------------------------------------------------------------
 template <int R, bool MULTI>
-__global__ void __launch_bounds__(256) k_pass_adj(float2* __restrict__ psi, float2* __restrict__ lam, PassArgs a) {
+__global__ void __launch_bounds__(512) k_pass_adj(float2* __restrict__ psi, float2* __restrict__ lam,
+                                                  const int* __restrict__ prog, const float2* __restrict__ mats,
+                                                  float* __restrict__ partials, PassArgs a) {
   extern __shared__ float2 lds[];
   float2* ldp = lds;
   const int T = blockDim.x;
   float2* ldl = lds + T * R;
   const int tid = threadIdx.x;
-  const TileCtx c = make_ctx(a, tid, MULTI);
-  const float2* mb = a.mats + (c.valid ? c.circuit : 0) * a.mat_stride;
-  load_tiles<R, MULTI>(psi, ldp, a, tid, T, c);
-  load_tiles<R, MULTI>(lam, ldl, a, tid, T, c);
+  const int lane = tid & 63;
+  const TileCtx c = make_ctx(a, prog, tid, MULTI);
+  const float2* mb = mats + (c.valid ? c.circuit : 0) * a.mat_stride;
+  load_tiles<R, MULTI>(psi, ldp, prog, a, tid, T, c);
+  load_tiles<R, MULTI>(lam, ldl, prog, a, tid, T, c);
   __syncthreads();
 
   // partial rows: W waves per tile when a tile spans several waves
@@ -373,33 +508,40 @@ __global__ void __launch_bounds__(256) k_pass_adj(float2* __restrict__ psi, floa
   const int tl = tid & (tpt - 1);
   int64_t tile_in_circuit = MULTI ? 0 : ((int64_t)blockIdx.x & ((1ll << (a.n - a.k)) - 1));
   const int64_t prow = tile_in_circuit * W + (tl >> 6);
-  const bool writer = c.valid && ((tid & (seg - 1)) == 0);
+  const bool seg_writer = c.valid && ((tid & (seg - 1)) == 0);
+  const bool wave_writer = (lane & 7) == 0;
+  float* const pbase = partials + (c.circuit * a.n_gslots * a.np + prow) * 8;
 
-  const int nst = a.prog[2];
+  const int nst = prog[2];
   for (int s = 0; s < nst; ++s) {
-    const int* st = a.prog + kHdrStages + kStageWords * s;
-    const int base = stage_base(tid, a.tT, a.k, st[2]);
+    const int* st = prog + kHdrStages + kStageWords * s;
+    const int sb = stage_base(tid, a.tT, a.k, st[2]);
     float2 vp[R], vl[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) {
-      vp[m] = ldp[swz(base + st[4 + m])];
-      vl[m] = ldl[swz(base + st[4 + m])];
+      vp[m] = ldp[sb ^ st[4 + m]];
+      vl[m] = ldl[sb ^ st[4 + m]];
     }
     const int nops = st[0];
-    const int* op = a.prog + st[1];
+    const int* op = prog + st[1];
     for (int o = 0; o < nops; ++o, op += 8) {
       const int type = op[0] & 0xff;
       if (type == kOpGrad1 || type == kOpCGrad1) {
         float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         op_grad<R>(vp, vl, op[1], op[3], acc);
+        float* dst = pbase + (int64_t)op[5] * a.np * 8;
+        if (seg == 64) {
+          const float tot = reduce8_wave(acc, lane);
+          if (wave_writer) dst[(lane >> 3) & 7] = tot;
+        } else {
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-          for (int off = 1; off < seg; off <<= 1) acc[q] += __shfl_xor(acc[q], off, 64);
-        if (writer) {
-          float4* dst = reinterpret_cast<float4*>(
-              a.partials + (((c.circuit * a.n_gslots + op[5]) * a.np) + prow) * 8);
-          dst[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
-          dst[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+          for (int q = 0; q < 8; ++q)
+            for (int off = 1; off < seg; off <<= 1) acc[q] += __shfl_xor(acc[q], off, 64);
+          if (seg_writer) {
+            float4* d4 = reinterpret_cast<float4*>(dst);
+            d4[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+            d4[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+          }
         }
       } else {
         apply_op<R, true>(vp, vl, op, mb);
@@ -407,28 +549,49 @@ __global__ void __launch_bounds__(256) k_pass_adj(float2* __restrict__ psi, floa
     }
 #pragma unroll
     for (int m = 0; m < R; ++m) {
-      ldp[swz(base + st[4 + m])] = vp[m];
-      ldl[swz(base + st[4 + m])] = vl[m];
+      ldp[sb ^ st[4 + m]] = vp[m];
+      ldl[sb ^ st[4 + m]] = vl[m];
     }
     __syncthreads();
   }
-  store_tiles<R, MULTI>(psi, ldp, a, tid, T, c);
-  store_tiles<R, MULTI>(lam, ldl, a, tid, T, c);
+  store_tiles<R, MULTI>(psi, ldp, prog, a, tid, T, c);
+  store_tiles<R, MULTI>(lam, ldl, prog, a, tid, T, c);
+}
+
+// gfx950 provides 160 KiB of LDS per workgroup; above 64 KiB the kernel must opt in once.
+constexpr size_t kMaxLds = 160 * 1024;
+
+template <typename K>
+static bool allow_lds(K kernel, size_t lds) {
+  if (lds <= 65536) return true;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds) == hipSuccess;
 }
 
 template <int R>
 static int launch_pass(bool adjoint, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t st, float2* psi,
-                       float2* lam, const PassArgs& a) {
+                       float2* lam, const int* prog, const float2* mats, float* partials, const PassArgs& a) {
+  bool ok = true;
   if (adjoint) {
-    if (multi)
-      hipLaunchKernelGGL((k_pass_adj<R, true>), grid, block, lds, st, psi, lam, a);
-    else
-      hipLaunchKernelGGL((k_pass_adj<R, false>), grid, block, lds, st, psi, lam, a);
+    if (multi) {
+      ok = allow_lds(k_pass_adj<R, true>, lds);
+      if (ok) hipLaunchKernelGGL((k_pass_adj<R, true>), grid, block, lds, st, psi, lam, prog, mats, partials, a);
+    } else {
+      ok = allow_lds(k_pass_adj<R, false>, lds);
+      if (ok) hipLaunchKernelGGL((k_pass_adj<R, false>), grid, block, lds, st, psi, lam, prog, mats, partials, a);
+    }
   } else {
-    if (multi)
-      hipLaunchKernelGGL((k_pass_fwd<R, true>), grid, block, lds, st, psi, a);
-    else
-      hipLaunchKernelGGL((k_pass_fwd<R, false>), grid, block, lds, st, psi, a);
+    if (multi) {
+      ok = allow_lds(k_pass_fwd<R, true>, lds);
+      if (ok) hipLaunchKernelGGL((k_pass_fwd<R, true>), grid, block, lds, st, psi, prog, mats, a);
+    } else {
+      ok = allow_lds(k_pass_fwd<R, false>, lds);
+      if (ok) hipLaunchKernelGGL((k_pass_fwd<R, false>), grid, block, lds, st, psi, prog, mats, a);
+    }
+  }
+  if (!ok) {
+    set_error("sv pass: could not enable %zu bytes of LDS", lds);
+    return kLaunchFailed;
   }
   return check_launch(adjoint ? "tedq_sv_adjoint_pass" : "tedq_sv_apply_pass");
 }
@@ -442,7 +605,7 @@ static int run_pass(bool adjoint, void* psi, void* lam, int64_t batch, int n, in
     return kBadArgument;
   }
   const int tT = k - r;
-  if (threads_log2 < tT || threads_log2 > 8 || (threads_log2 > tT && k != n)) {
+  if (threads_log2 < tT || threads_log2 > 9 || (threads_log2 > tT && k != n)) {
     set_error("sv pass: threads_log2=%d incompatible with k=%d r=%d n=%d", threads_log2, k, r, n);
     return kBadArgument;
   }
@@ -453,8 +616,6 @@ static int run_pass(bool adjoint, void* psi, void* lam, int64_t batch, int n, in
     return kBadArgument;
   }
   PassArgs a;
-  a.prog = dev_prog;
-  a.mats = static_cast<const float2*>(mats);
   a.mat_stride = mat_stride;
   a.batch = batch;
   a.total_tiles = batch << (n - k);
@@ -463,7 +624,6 @@ static int run_pass(bool adjoint, void* psi, void* lam, int64_t batch, int n, in
   a.k = k;
   a.tT = tT;
   a.gT = threads_log2 - tT;
-  a.partials = partials;
   a.np = np;
   a.n_gslots = n_gslots;
   const int T = 1 << threads_log2;
@@ -480,20 +640,21 @@ static int run_pass(bool adjoint, void* psi, void* lam, int64_t batch, int n, in
     return kBadArgument;
   }
   const size_t lds = (size_t)T * R * sizeof(float2) * (adjoint ? 2 : 1);
-  if (lds > 65536) {
-    set_error("sv pass: LDS request %zu bytes exceeds 64 KiB", lds);
+  if (lds > kMaxLds) {
+    set_error("sv pass: LDS request %zu bytes exceeds %zu", lds, kMaxLds);
     return kBadArgument;
   }
   float2* p = static_cast<float2*>(psi);
   float2* l = static_cast<float2*>(lam);
+  const float2* mt = static_cast<const float2*>(mats);
   hipStream_t st = static_cast<hipStream_t>(stream);
   dim3 grid((unsigned)blocks), block((unsigned)T);
   switch (r) {
-    case 1: return launch_pass<2>(adjoint, multi, grid, block, lds, st, p, l, a);
-    case 2: return launch_pass<4>(adjoint, multi, grid, block, lds, st, p, l, a);
-    case 3: return launch_pass<8>(adjoint, multi, grid, block, lds, st, p, l, a);
-    case 4: return launch_pass<16>(adjoint, multi, grid, block, lds, st, p, l, a);
-    case 5: return launch_pass<32>(adjoint, multi, grid, block, lds, st, p, l, a);
+    case 1: return launch_pass<2>(adjoint, multi, grid, block, lds, st, p, l, dev_prog, mt, partials, a);
+    case 2: return launch_pass<4>(adjoint, multi, grid, block, lds, st, p, l, dev_prog, mt, partials, a);
+    case 3: return launch_pass<8>(adjoint, multi, grid, block, lds, st, p, l, dev_prog, mt, partials, a);
+    case 4: return launch_pass<16>(adjoint, multi, grid, block, lds, st, p, l, dev_prog, mt, partials, a);
+    case 5: return launch_pass<32>(adjoint, multi, grid, block, lds, st, p, l, dev_prog, mt, partials, a);
     default: return kBadArgument;
   }
 }

@@ -56,11 +56,10 @@ class _AdjointFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, executor, rows):  # pylint: disable=arguments-differ
-        psi, mats, amats = executor.run_forward(rows, need_grad=True)
+        psi = executor.run_forward(rows)
         outs = executor.measure(psi)
         ctx.executor = executor
-        ctx.row_dtype = rows.dtype
-        ctx.save_for_backward(psi, mats, amats)
+        ctx.save_for_backward(rows, psi)
         res = []
         for spec, o in zip(executor.specs, outs):
             if spec.kind in ("zexp", "fexp"):
@@ -73,10 +72,10 @@ class _AdjointFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *grads):  # pylint: disable=arguments-differ
-        psi, mats, amats = ctx.saved_tensors
+        rows, psi = ctx.saved_tensors
         ex = ctx.executor
         lam = ex.seed_adjoint(psi, grads)
-        grad = ex.run_backward(psi.clone(), lam, amats, mats, grad_dtype=ctx.row_dtype)
+        grad = ex.run_backward(rows, psi.clone(), lam, grad_dtype=rows.dtype)
         return None, grad
 
 
@@ -117,6 +116,8 @@ class PyTorchBackend(CompiledCircuit):
         self._diff_method = kwargs.get("diff_method", "back_prop")
         self._tile_bits = kwargs.get("tile_bits", 12)
         self._reg_bits = kwargs.get("reg_bits", 4)
+        self._tile_bits_bwd = kwargs.get("tile_bits_bwd", None)
+        self._reg_bits_bwd = kwargs.get("reg_bits_bwd", None)
         if self._interface != "pytorch":
             raise ValueError(f"{self._interface}: pytroch_backend only supports pytorch interface!")
         self._device = None
@@ -191,7 +192,8 @@ class PyTorchBackend(CompiledCircuit):
                 if self._init_state:
                     init = self._init_state.matrix
                 self._executor = SVExecutor(n, units, n_params, pool, self._measurements, init_state=init,
-                                            k_max=self._tile_bits, r=self._reg_bits)
+                                            k_max=self._tile_bits, r=self._reg_bits,
+                                            k_bwd=self._tile_bits_bwd, r_bwd=self._reg_bits_bwd)
         return self._executor
 
     def _flatten_and_update(self, params):
@@ -213,7 +215,7 @@ class PyTorchBackend(CompiledCircuit):
     def _run_rows(self, rows):
         """No-autograd evaluation of [B, P] rows -> list of per-measurement outputs [B, ...]."""
         ex = self.executor()
-        psi, _, _ = ex.run_forward(rows, need_grad=False)
+        psi = ex.run_forward(rows)
         outs = ex.measure(psi)
         res = []
         for spec, o in zip(ex.specs, outs):
@@ -230,7 +232,7 @@ class PyTorchBackend(CompiledCircuit):
             outs = _AdjointFunction.apply(ex, rows)
         else:
             with torch.no_grad():
-                psi, _, _ = ex.run_forward(rows, need_grad=False)
+                psi = ex.run_forward(rows)
                 outs = ex.measure(psi)
                 outs = [o.to(torch.float32) if s.kind in ("zexp", "fexp") else o for s, o in zip(ex.specs, outs)]
         results = [o[0] for o in outs]
@@ -307,7 +309,7 @@ class PyTorchBackend(CompiledCircuit):
         if not wanted or not isinstance(self.executor(), SVExecutor):
             return
         with torch.no_grad():
-            psi = self.executor().run_forward(rows.detach(), need_grad=False)[0][0]
+            psi = self.executor().run_forward(rows.detach())[0]
         n = self._num_qubits
         state = psi.reshape([2] * n)
         for meas in wanted:

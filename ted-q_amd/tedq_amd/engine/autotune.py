@@ -1,0 +1,59 @@
+"""Tile-shape autotuner for the state-vector passes.
+
+For a compiled circuit and a representative parameter batch, times the
+forward program and the adjoint program for a set of (tile bits k, register
+bits r) shapes with device events and configures the executor with the
+fastest of each.  The shapes trade pass count (larger k: fewer passes) against
+LDS per workgroup (2^k x 8 B forward, 2^k x 16 B adjoint) and register
+pressure (larger r: fewer LDS gather/scatter stages, more VGPRs).
+"""
+
+import torch
+
+FWD_SHAPES = ((12, 4), (13, 4), (13, 5), (12, 5), (11, 4), (12, 3))
+BWD_SHAPES = ((12, 4), (12, 3), (13, 4), (11, 4), (11, 3))
+
+
+def valid_shape(k, r, adjoint):
+    """At most 512 threads per tile, LDS within 160 KiB, at most 16 registers per state in the adjoint."""
+    return k - r <= 9 and (2 ** k) * (16 if adjoint else 8) <= 160 * 1024 and (not adjoint or r <= 4)
+
+
+def _time(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        fn()
+    t1.record()
+    t1.synchronize()
+    return t0.elapsed_time(t1) / reps
+
+
+def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2):
+    """Configure ``ex`` (an SVExecutor) with the fastest shapes for ``rows`` ([B, P] on the GPU).
+
+    Returns {"fwd": {shape: ms}, "bwd": {shape: ms}, "best": (k, r, k_bwd, r_bwd)}.
+    """
+    n = ex.n
+    fwd_shapes = [s for s in fwd_shapes if s[0] <= n and valid_shape(*s, False)] or [(n, min(4, n))]
+    bwd_shapes = [s for s in bwd_shapes if s[0] <= n and valid_shape(*s, True)] or [(n, min(4, n))]
+    k0, r0 = ex.fwd.plan.k, ex.fwd.plan.r
+    kb0, rb0 = ex.bwd.plan.k, ex.bwd.plan.r
+    fwd_ms, bwd_ms = {}, {}
+    for k, r in fwd_shapes:
+        ex.configure(k, r, kb0, rb0)
+        fwd_ms[(k, r)] = _time(lambda: ex.run_forward(rows), reps)
+    kf, rf = min(fwd_ms, key=fwd_ms.get)
+    psi = ex.run_forward(rows)
+    ups = [torch.ones(rows.shape[0], device=rows.device) for _ in ex.specs]
+    lam = ex.seed_adjoint(psi, ups)
+    for k, r in bwd_shapes:
+        ex.configure(kf, rf, k, r)
+        bwd_ms[(k, r)] = _time(lambda: ex.run_backward(rows, psi.clone(), lam.clone()), reps)
+    kb, rb = min(bwd_ms, key=bwd_ms.get)
+    ex.configure(kf, rf, kb, rb)
+    del k0, r0
+    return {"fwd": fwd_ms, "bwd": bwd_ms, "best": (kf, rf, kb, rb)}

@@ -72,8 +72,8 @@ class Slot:
 @dataclass
 class Unit:
     """A gate as the kernels see it: physical bits it touches and how to apply it."""
-    kind: str              # "u1", "cu1", "cx", "cz", "u2"
-    bits: tuple            # for u1/cu1/cx: (target, *controls); u2: (msb, lsb); cz: all bits
+    kind: str              # "u1", "cu1", "cx", "cz"
+    bits: tuple            # for u1/cu1/cx: (target, *controls); cz: all bits
     comp: Component = None
     index: int = 0
 
@@ -113,11 +113,10 @@ def gate_to_units(n, gate, srcs, const_pool):
     def comp(c, s=(), mat=-1):
         return Component(c, dagger, tuple(s), mat)
 
+    # a user matrix already reflects ``inverse()`` (GateBase.inverse daggers it)
     if code in _ONEQ_CODES and len(bits) == 1:
         if code == 23:
-            off = len(const_pool)
-            const_pool.extend(np.asarray(gate.matrix, dtype=np.complex128).reshape(-1).tolist())
-            return [Unit("u1", (bits[0],), comp(23, (), off))]
+            return [Unit("u1", (bits[0],), _const_comp(gate.matrix, const_pool))]
         return [Unit("u1", (bits[0],), comp(code, srcs))]
     if name == "CNOT":
         return [Unit("cx", (bits[1], bits[0]))]
@@ -137,10 +136,41 @@ def gate_to_units(n, gate, srcs, const_pool):
         block = {"CRX": 14, "CRY": 15, "CRZ": 16, "ControlledPhaseShift": 18}[name]
         return [Unit("cu1", (bits[1], bits[0]), comp(block, srcs))]
     if code == 23 and len(bits) == 2:
-        off = len(const_pool)
-        const_pool.extend(np.asarray(gate.matrix, dtype=np.complex128).reshape(-1).tolist())
-        return [Unit("u2", (bits[0], bits[1]), comp(23, (), off))]
+        return two_qubit_units(np.asarray(gate.matrix, dtype=np.complex128).reshape(4, 4), bits[0], bits[1],
+                               const_pool)
     raise NotImplementedError(f"{gate.name} on {len(bits)} qubits is not supported by the SV engine")
+
+
+def _const_comp(mat, const_pool):
+    off = len(const_pool)
+    const_pool.extend(np.asarray(mat, dtype=np.complex128).reshape(-1).tolist())
+    return Component(23, False, (), off)
+
+
+def _multiplexor(target, control, m0, m1, const_pool):
+    """Apply m0 to ``target`` where ``control`` is 0 and m1 where it is 1."""
+    return [Unit("cu1", (target, control), _const_comp(m1, const_pool)),
+            Unit("cx", (control,)),
+            Unit("cu1", (target, control), _const_comp(m0, const_pool)),
+            Unit("cx", (control,))]
+
+
+def two_qubit_units(m, b_hi, b_lo, const_pool):
+    """Exact lowering of a 4x4 operator on physical bits (b_hi = first qubit, b_lo = second).
+
+    Cosine-sine decomposition m = diag(U0, U1) . CS . diag(V0, V1), where the
+    diagonal blocks act on the second qubit multiplexed by the first and CS is
+    a rotation of the first qubit multiplexed by the second; each multiplexor
+    is two controlled single-qubit ops.
+    """
+    from scipy.linalg import cossin  # pylint: disable=import-outside-toplevel
+    u, cs, vdh = cossin(m, p=2, q=2)
+    units = _multiplexor(b_lo, b_hi, vdh[:2, :2], vdh[2:, 2:], const_pool)
+    r0 = np.array([[cs[0, 0], cs[0, 2]], [cs[2, 0], cs[2, 2]]])
+    r1 = np.array([[cs[1, 1], cs[1, 3]], [cs[3, 1], cs[3, 3]]])
+    units += _multiplexor(b_hi, b_lo, r0, r1, const_pool)
+    units += _multiplexor(b_lo, b_hi, u[:2, :2], u[2:, 2:], const_pool)
+    return units
 
 
 def _schedule(units, order, max_bits, seed_bits, allowed_bits=None):
@@ -261,9 +291,6 @@ def _build_pass(units, pass_units, tile_bits, r, slots):
             elif u.kind == "cz":
                 mask = sum(1 << reg_of[c] for c in u.bits)
                 stage.ops.append(("cz", 0, 0, mask, -1))
-            elif u.kind == "u2":
-                slots.append(Slot(SLOT_2Q, [u.comp]))
-                stage.ops.append(("u2", reg_of[u.bits[0]], reg_of[u.bits[1]], 0, len(slots) - 1))
             for b in u.bits:
                 last_on_bit[b] = len(stage.ops) - 1
         p.stages.append(stage)

@@ -9,7 +9,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_U2, OP_GRAD1, OP_CGRAD1, OP_CD1,
+from .planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1,
                       OP_DAGGER, SLOT_1Q)
 
 HDR_TILE_BITS = 4
@@ -33,6 +33,11 @@ class EncodedPlan:
     pgslot: np.ndarray        # int32 [n_params]: gradient slot of each trainable parameter
 
 
+def lds_swizzle(e):
+    """LDS element swizzle of the pass kernels (csrc/tedq_common.h ``swz``); an XOR-linear involution."""
+    return e ^ (((e >> 4) ^ (e >> 8)) & 15)
+
+
 def _pdep(v, positions):
     out = 0
     for i, p in enumerate(positions):
@@ -52,8 +57,6 @@ def _fwd_op_words(op, slots):
         t = OP_CX
     elif kind == "cz":
         t = OP_CZ
-    elif kind == "u2":
-        t = OP_U2
     else:
         raise ValueError(kind)
     return [t, b0, b1, cmask, mat, -1, 0, 0]
@@ -100,7 +103,7 @@ def _encode_pass(p, k, r, slots, adjoint):
         header[base + 2] = ((1 << k) - 1) & ~regmask
         header[base + 3] = regmask
         for m in range(1 << r):
-            header[base + 4 + m] = _pdep(m, st.reg_bits)
+            header[base + 4 + m] = lds_swizzle(_pdep(m, st.reg_bits))
         op_words.extend(ops)
     words = header.astype(np.int32)
     words[HDR_HI_OFF:HDR_HI_OFF + 64] = hi.view(np.int32)

@@ -90,29 +90,110 @@ def compile_measurements(n, measurements):
 
 
 class _DeviceBuffers:
-    def __init__(self, enc, zmasks, device):
+    def __init__(self, enc, device):
         self.prog = torch.from_numpy(enc.prog).to(device)
         self.slot_recs = torch.from_numpy(enc.slot_recs.reshape(-1)).to(device)
         self.comp_recs = torch.from_numpy(enc.comp_recs.reshape(-1)).to(device)
         self.const_pool = torch.from_numpy(enc.const_pool.reshape(-1)).to(device)
         self.pgslot = torch.from_numpy(enc.pgslot).to(device)
-        self.zmasks = torch.tensor(np.asarray(zmasks, dtype=np.uint64).view(np.int64), device=device) \
-            if zmasks else None
         self.prog_base = self.prog.data_ptr()
+        # unknown-gate-code flag of the table builder (set once; never cleared inside a graph)
+        self.err = torch.zeros(1, dtype=torch.int32, device=device)
 
     def pass_ptr(self, word_offset):
         import ctypes  # pylint: disable=import-outside-toplevel
         return ctypes.c_void_p(self.prog_base + 4 * word_offset)
 
 
-class SVExecutor:
-    """Compiled SV program for one circuit structure."""
+class _Program:
+    """One fusion plan (tile bits k, register bits r) with its encoded device program."""
 
-    def __init__(self, n, units, n_params, const_pool, measurements, init_state=None, k_max=12, r=4):
+    def __init__(self, n, units, n_params, const_pool, k, r):
         self.n = n
         self.n_params = n_params
-        self.plan = build_plan(n, units, n_params, const_pool, k_max=k_max, r=r)
+        self.plan = build_plan(n, units, n_params, const_pool, k_max=k, r=r)
         self.enc = encode_plan(self.plan)
+        self._dev = {}
+
+    def buffers(self, device):
+        key = (device.type, device.index)
+        if key not in self._dev:
+            self._dev[key] = _DeviceBuffers(self.enc, device)
+        return self._dev[key]
+
+    def threads_log2(self):
+        k, r = self.plan.k, self.plan.r
+        tT = k - r
+        if k < self.n:
+            return tT
+        return max(tT, min(8, max(6, tT)))
+
+    def partial_rows(self):
+        k, r = self.plan.k, self.plan.r
+        tpt = 1 << (k - r)
+        w = max(1, tpt // 64)
+        return (1 << (self.n - k)) * w
+
+    def tables(self, params, need_grad):
+        """Slot matrices (and A matrices) for [B, P] rows."""
+        dev = params.device
+        buf = self.buffers(dev)
+        B = params.shape[0]
+        mats = torch.empty((B, self.plan.mat_stride, 2), dtype=torch.float32, device=dev)
+        # every trainable column's A matrix is written by the table kernel; other
+        # columns are never read (their gradient slot is -1)
+        amats = torch.empty((B, max(self.n_params, 1), 4, 2), dtype=torch.float64, device=dev) if need_grad else None
+        params = params.contiguous()
+        hip.call("tedq_sv_build_tables", hip.ptr(params) if self.n_params else None,
+                 int(params.dtype == torch.float64), B, self.n_params, hip.ptr(buf.slot_recs),
+                 len(self.plan.slots), hip.ptr(buf.comp_recs), hip.ptr(buf.const_pool), hip.ptr(mats),
+                 self.plan.mat_stride, hip.ptr(amats), hip.ptr(buf.err), hip.stream_handle(dev))
+        return mats, amats
+
+    def forward_passes(self, psi, mats, probe=None):
+        dev = psi.device
+        buf = self.buffers(dev)
+        B = psi.shape[0]
+        st = hip.stream_handle(dev)
+        tl = self.threads_log2()
+        for off, mask in zip(self.enc.fwd_offsets, self.enc.tile_masks):
+            if probe:
+                probe.begin("sv_pass_fwd")
+            hip.call("tedq_sv_apply_pass", hip.ptr(psi), B, self.n, self.plan.k, self.plan.r, tl,
+                     buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, st)
+            if probe:
+                probe.end("sv_pass_fwd")
+
+    def adjoint_passes(self, psi, lam, mats, partials, probe=None):
+        dev = psi.device
+        buf = self.buffers(dev)
+        B = psi.shape[0]
+        st = hip.stream_handle(dev)
+        tl = self.threads_log2()
+        rows = self.partial_rows()
+        G = self.plan.n_gslots
+        for off, mask in zip(self.enc.adj_offsets, reversed(self.enc.tile_masks)):
+            if probe:
+                probe.begin("sv_pass_adj")
+            hip.call("tedq_sv_adjoint_pass", hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r, tl,
+                     buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, st)
+            if probe:
+                probe.end("sv_pass_adj")
+
+
+class SVExecutor:
+    """Compiled SV program for one circuit structure.
+
+    The forward and the adjoint sweep may use different tile / register
+    shapes (``k``, ``r`` and ``k_bwd``, ``r_bwd``), each with its own fusion plan.
+    """
+
+    def __init__(self, n, units, n_params, const_pool, measurements, init_state=None, k_max=12, r=4,
+                 k_bwd=None, r_bwd=None):
+        self.n = n
+        self.n_params = n_params
+        self._units = units
+        self._pool = list(const_pool)
         self.specs, self.zmasks = compile_measurements(n, measurements)
         self.init_state = None
         if init_state is not None:
@@ -123,78 +204,66 @@ class SVExecutor:
         self._dev = {}
         # optional timing probe with begin(kind)/end(kind) around every pass launch
         self.probe = None
+        self.configure(k_max, r, k_bwd, r_bwd)
+
+    def configure(self, k, r, k_bwd=None, r_bwd=None):
+        """(Re)build the forward and adjoint programs for the given tile / register shapes."""
+        k_bwd = k if k_bwd is None else k_bwd
+        # the adjoint holds psi and lambda registers: more than 16 amplitudes each would spill
+        r_bwd = min(4, r if r_bwd is None else r_bwd)
+        self.fwd = _Program(self.n, self._units, self.n_params, list(self._pool), k, r)
+        if (k_bwd, r_bwd) == (k, r):
+            self.bwd = self.fwd
+        else:
+            self.bwd = _Program(self.n, self._units, self.n_params, list(self._pool), k_bwd, r_bwd)
 
     # ------------------------------------------------------------------
     @property
+    def plan(self):
+        return self.fwd.plan
+
+    @property
+    def enc(self):
+        return self.fwd.enc
+
+    @property
     def n_passes(self):
-        return len(self.plan.passes)
-
-    def _buffers(self, device):
-        key = (device.type, device.index)
-        if key not in self._dev:
-            buf = _DeviceBuffers(self.enc, self.zmasks, device)
-            if self.init_state is not None:
-                buf.init = torch.from_numpy(self.init_state.view(np.float32).copy()).to(device)
-            self._dev[key] = buf
-        return self._dev[key]
-
-    def _threads_log2(self, adjoint=False):
-        k, r = self.plan.k, self.plan.r
-        tT = k - r
-        if k < self.n:
-            return tT
-        cap = 8 if (not adjoint or r <= 4) else 7
-        return max(tT, min(cap, max(6, tT)))
+        return len(self.fwd.plan.passes)
 
     def partial_rows(self):
-        k, r = self.plan.k, self.plan.r
-        tpt = 1 << (k - r)
-        w = max(1, tpt // 64)
-        return (1 << (self.n - k)) * w
+        return self.bwd.partial_rows()
+
+    def _zmask_buffer(self, device):
+        key = (device.type, device.index)
+        if key not in self._dev:
+            z = torch.tensor(np.asarray(self.zmasks, dtype=np.uint64).view(np.int64), device=device) \
+                if self.zmasks else None
+            init = None
+            if self.init_state is not None:
+                init = torch.from_numpy(self.init_state.view(np.float32).copy()).to(device)
+            self._dev[key] = (z, init)
+        return self._dev[key]
 
     # ------------------------------------------------------------------
-    def build_tables(self, params, need_grad):
-        """params: [B, P] float32/float64 on the device."""
+    def run_forward(self, params, need_grad=False):  # pylint: disable=unused-argument
+        """Apply the circuit to every parameter row.  Returns psi [B, 2^n] complex64."""
         dev = params.device
-        buf = self._buffers(dev)
         B = params.shape[0]
-        mats = torch.empty((B, self.plan.mat_stride, 2), dtype=torch.float32, device=dev)
-        amats = torch.zeros((B, max(self.n_params, 1), 4, 2), dtype=torch.float64, device=dev) if need_grad else None
-        err = torch.zeros(1, dtype=torch.int32, device=dev)
-        params = params.contiguous()
-        hip.call("tedq_sv_build_tables", hip.ptr(params) if self.n_params else None,
-                 int(params.dtype == torch.float64), B, self.n_params, hip.ptr(buf.slot_recs),
-                 len(self.plan.slots), hip.ptr(buf.comp_recs), hip.ptr(buf.const_pool), hip.ptr(mats),
-                 self.plan.mat_stride, hip.ptr(amats), hip.ptr(err), hip.stream_handle(dev))
-        return mats, amats
-
-    def run_forward(self, params, need_grad=False):
-        """Apply the circuit to every parameter row.  Returns (psi [B, 2^n] complex64, mats, amats)."""
-        dev = params.device
-        buf = self._buffers(dev)
-        B = params.shape[0]
-        mats, amats = self.build_tables(params, need_grad)
+        mats, _ = self.fwd.tables(params, False)
         psi = torch.empty((B, 2 ** self.n), dtype=torch.complex64, device=dev)
         st = hip.stream_handle(dev)
-        if self.init_state is None:
+        _, init = self._zmask_buffer(dev)
+        if init is None:
             hip.call("tedq_sv_init_zero", hip.ptr(psi), B, self.n, st)
         else:
-            hip.call("tedq_sv_init_copy", hip.ptr(psi), hip.ptr(buf.init), B, self.n, st)
-        tl = self._threads_log2()
-        probe = self.probe
-        for off, mask in zip(self.enc.fwd_offsets, self.enc.tile_masks):
-            if probe:
-                probe.begin("sv_pass_fwd")
-            hip.call("tedq_sv_apply_pass", hip.ptr(psi), B, self.n, self.plan.k, self.plan.r, tl,
-                     buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, st)
-            if probe:
-                probe.end("sv_pass_fwd")
-        return psi, mats, amats
+            hip.call("tedq_sv_init_copy", hip.ptr(psi), hip.ptr(init), B, self.n, st)
+        self.fwd.forward_passes(psi, mats, self.probe)
+        return psi
 
     def measure(self, psi):
         """List of per-measurement outputs, each [B, *shape] (expvals fp64, probs fp32, state c64)."""
         dev = psi.device
-        buf = self._buffers(dev)
+        zbuf, _ = self._zmask_buffer(dev)
         B = psi.shape[0]
         st = hip.stream_handle(dev)
         nchunk = hip.load().tedq_sv_reduce_chunks(self.n)
@@ -203,7 +272,7 @@ class SVExecutor:
             M = len(self.zmasks)
             partial = torch.empty((B, nchunk, M), dtype=torch.float64, device=dev)
             zout = torch.empty((B, M), dtype=torch.float64, device=dev)
-            hip.call("tedq_sv_expval_z", hip.ptr(psi), B, self.n, hip.ptr(buf.zmasks), M, hip.ptr(partial),
+            hip.call("tedq_sv_expval_z", hip.ptr(psi), B, self.n, hip.ptr(zbuf), M, hip.ptr(partial),
                      hip.ptr(zout), st)
         outs = []
         for spec in self.specs:
@@ -235,19 +304,27 @@ class SVExecutor:
     def seed_adjoint(self, psi, grads):
         """lambda = sum_m (d out_m)^* seed_m applied to psi; grads aligned with self.specs (None = zero)."""
         dev = psi.device
-        buf = self._buffers(dev)
+        zbuf, _ = self._zmask_buffer(dev)
         B = psi.shape[0]
         st = hip.stream_handle(dev)
         lam = torch.empty_like(psi)
         M = len(self.zmasks)
         dy = None
         if M:
-            dy = torch.zeros((B, M), dtype=torch.float64, device=dev)
+            cols = [None] * M
             for spec, g in zip(self.specs, grads):
                 if spec.kind == "zexp" and g is not None:
-                    dy[:, spec.zindex] += g.to(torch.float64).reshape(B)
-        hip.call("tedq_sv_lambda_z", hip.ptr(psi), hip.ptr(lam), B, self.n,
-                 hip.ptr(buf.zmasks) if M else None, M, hip.ptr(dy), st)
+                    g = g.to(torch.float64).reshape(B)
+                    cols[spec.zindex] = g if cols[spec.zindex] is None else cols[spec.zindex] + g
+            zero = None
+            for i, c in enumerate(cols):
+                if c is None:
+                    if zero is None:
+                        zero = torch.full((B,), 0.0, dtype=torch.float64, device=dev)
+                    cols[i] = zero
+            dy = torch.stack(cols, 1).contiguous()
+        hip.call("tedq_sv_lambda_z", hip.ptr(psi), hip.ptr(lam), B, self.n, hip.ptr(zbuf) if M else None, M,
+                 hip.ptr(dy), st)
         for spec, g in zip(self.specs, grads):
             if g is None or spec.kind == "zexp":
                 continue
@@ -267,31 +344,46 @@ class SVExecutor:
                 hip.call("tedq_sv_axpy", hip.ptr(lam), hip.ptr(gs), B, self.n, 0.5, st)
         return lam
 
-    def run_backward(self, psi, lam, amats, mats, grad_dtype=torch.float32):
-        """Adjoint sweep; consumes psi and lam (both are overwritten).  Returns grad [B, P]."""
+    def run_backward(self, params, psi, lam, grad_dtype=torch.float32):
+        """Adjoint sweep for [B, P] rows; consumes psi and lam (both are overwritten).  Returns grad [B, P]."""
         dev = psi.device
-        buf = self._buffers(dev)
         B = psi.shape[0]
-        st = hip.stream_handle(dev)
-        G = self.plan.n_gslots
-        rows = self.partial_rows()
         grad = torch.empty((B, max(self.n_params, 1)), dtype=grad_dtype, device=dev)
         if self.n_params == 0:
             return grad[:, :0]
+        prog = self.bwd
+        mats, amats = prog.tables(params, True)
+        G = prog.plan.n_gslots
+        rows = prog.partial_rows()
         partials = torch.empty((B, max(G, 1), rows, 8), dtype=torch.float32, device=dev)
-        tl = self._threads_log2(adjoint=True)
-        probe = self.probe
-        for off, mask in zip(self.enc.adj_offsets, reversed(self.enc.tile_masks)):
-            if probe:
-                probe.begin("sv_pass_adj")
-            hip.call("tedq_sv_adjoint_pass", hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r, tl,
-                     buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, st)
-            if probe:
-                probe.end("sv_pass_adj")
+        prog.adjoint_passes(psi, lam, mats, partials, self.probe)
         C = torch.empty((B, max(G, 1), 8), dtype=torch.float64, device=dev)
+        buf = prog.buffers(dev)
         hip.call("tedq_sv_grad_reduce", hip.ptr(partials), B, G, rows, hip.ptr(amats), hip.ptr(buf.pgslot),
-                 self.n_params, hip.ptr(C), hip.ptr(grad), int(grad_dtype == torch.float64), st)
+                 self.n_params, hip.ptr(C), hip.ptr(grad), int(grad_dtype == torch.float64),
+                 hip.stream_handle(dev))
         return grad
+
+    def value_and_grad(self, rows, grads_out):
+        """Measurements and parameter gradient for [B, P] rows in one launch sequence, without autograd.
+
+        ``grads_out`` holds one upstream gradient per measurement (None = 0).
+        Returns (list of measurement outputs, grad [B, P]).
+        """
+        psi = self.run_forward(rows)
+        outs = self.measure(psi)
+        lam = self.seed_adjoint(psi, grads_out)
+        grad = self.run_backward(rows, psi, lam, grad_dtype=rows.dtype)
+        return outs, grad
+
+    def memory_bytes(self, batch, with_grad=True):
+        """Device bytes one launch sequence of ``batch`` rows needs (states, tables, partials)."""
+        amps = batch * (2 ** self.n) * 8
+        tables = batch * (self.fwd.plan.mat_stride + self.bwd.plan.mat_stride) * 8 + batch * max(self.n_params, 1) * 64
+        if not with_grad:
+            return amps + tables
+        partials = batch * max(self.bwd.plan.n_gslots, 1) * self.bwd.partial_rows() * 32
+        return 2 * amps + tables + partials
 
 
 def _factored_args(spec, device):  # pylint: disable=unused-argument

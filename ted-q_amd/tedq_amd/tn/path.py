@@ -1,0 +1,111 @@
+"""Greedy pairwise contraction order for the circuit tensor networks.
+
+The reference obtains its order from external optimisers (opt_einsum
+``contract_expression(optimize='auto')`` in tedq/oe_wrapper/oe_wrapper.py:57,
+JDOptTN or cotengra, tedq/backends/compiled_circuit.py:340-404), none of which
+is available here, so the order is this module's own, deterministic greedy
+rule, and a computed path is frozen with the network it belongs to:
+
+* candidates are pairs of live tensors that share at least one index;
+* the pair minimising ``size(result) - size(a) - size(b)`` is contracted
+  first (sizes are element counts, every index has dimension 2); ties go to
+  the smaller ``(min(id), max(id))`` pair of SSA ids;
+* an index is summed when it occurs in the two operands, in no other live
+  tensor and not in the requested output;
+* when no pair shares an index, the remaining tensors are joined by outer
+  products, smallest first.
+
+The result is an SSA path: step s contracts ids (i, j) and creates id
+``n_inputs + s``.
+"""
+
+import heapq
+
+
+def _cost(a, b, keep_counts, out_set):
+    union = a | b
+    summed = {ix for ix in (a & b) if keep_counts[ix] == 2 and ix not in out_set}
+    res = union - summed
+    return (1 << len(res)) - (1 << len(a)) - (1 << len(b)), frozenset(res)
+
+
+def greedy_path(inputs, output):
+    """SSA contraction path and the index set of every intermediate.
+
+    ``inputs``: list of index lists; ``output``: list of kept indices.
+    Returns (path, sets) where ``sets[id]`` is the frozenset of indices of
+    tensor ``id`` (inputs first, then one per step).
+    """
+    out_set = set(output)
+    sets = [frozenset(t) for t in inputs]
+    counts = {}
+    for t in sets:
+        for ix in t:
+            counts[ix] = counts.get(ix, 0) + 1
+    holders = {}
+    for i, t in enumerate(sets):
+        for ix in t:
+            holders.setdefault(ix, set()).add(i)
+    live = set(range(len(sets)))
+    heap = []
+
+    def push(i, j):
+        if i > j:
+            i, j = j, i
+        c, _ = _cost(sets[i], sets[j], counts, out_set)
+        heapq.heappush(heap, (c, i, j))
+
+    seen = set()
+    for ix, hs in holders.items():
+        hs = sorted(hs)
+        for a in range(len(hs)):
+            for b in range(a + 1, len(hs)):
+                if (hs[a], hs[b]) not in seen:
+                    seen.add((hs[a], hs[b]))
+                    push(hs[a], hs[b])
+    path = []
+    while heap:
+        c, i, j = heapq.heappop(heap)
+        if i not in live or j not in live:
+            continue
+        _, res = _cost(sets[i], sets[j], counts, out_set)
+        new = len(sets)
+        sets.append(res)
+        path.append((i, j))
+        live -= {i, j}
+        for ix in sets[i] | sets[j]:
+            holders[ix].discard(i)
+            holders[ix].discard(j)
+            counts[ix] -= (ix in sets[i]) + (ix in sets[j])
+        for ix in res:
+            holders[ix].add(new)
+            counts[ix] += 1
+        live.add(new)
+        neigh = set()
+        for ix in res:
+            neigh |= holders[ix]
+        neigh.discard(new)
+        for m in sorted(neigh):
+            push(m, new)
+    # outer products of what is left, smallest first
+    rest = sorted(live, key=lambda t: (len(sets[t]), t))
+    while len(rest) > 1:
+        i, j = rest[0], rest[1]
+        _, res = _cost(sets[i], sets[j], counts, out_set)
+        new = len(sets)
+        sets.append(res)
+        path.append((min(i, j), max(i, j)))
+        rest = sorted(rest[2:] + [new], key=lambda t: (len(sets[t]), t))
+    return path, sets
+
+
+def path_stats(path, sets, n_inputs):
+    """log2 of the largest intermediate and log10 of the multiply-add count (complex MACs)."""
+    import math  # pylint: disable=import-outside-toplevel
+    largest = 0
+    flops = 0
+    for s, (i, j) in enumerate(path):
+        res = sets[n_inputs + s]
+        largest = max(largest, len(res))
+        flops += 1 << len(sets[i] | sets[j])
+    return largest, math.log10(max(flops, 1))

@@ -157,7 +157,7 @@ def run_pass(words, n, states, mats, acc):
             st = words[P.HDR_STAGES + P.STAGE_WORDS * si:P.HDR_STAGES + P.STAGE_WORDS * (si + 1)]
             nops, ostart, nonreg = int(st[0]), int(st[1]), int(st[2])
             nonreg_bits = [b for b in range(k) if (nonreg >> b) & 1]
-            reg_off = np.array([int(x) for x in st[4:4 + R]], dtype=np.int64)
+            reg_off = np.array([P.lds_swizzle(int(x)) for x in st[4:4 + R]], dtype=np.int64)
             thread_base = np.array([_pdep(tl, nonreg_bits) for tl in range(1 << (k - r))], dtype=np.int64)
             idx = thread_base[:, None] + reg_off[None, :]
             regs = [tile[idx] for tile in tiles]

@@ -1,0 +1,112 @@
+"""Continuous batching: every result is the same alone and in the mix.
+
+Tolerance: expectation values and gradients of a request served inside a
+batch of other requests (sharing launches with same-structure circuits and
+overlapping on other streams) must match the request evaluated alone to
+2e-6 absolute (single-precision rounding of a few reordered reductions), and
+match the complex128 oracle to 1e-5.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import tedq_amd as qai
+from oracle import sv_oracle as O
+from tedq_amd.serving.engine import Request, ServingEngine, latency_stats
+
+pytestmark = pytest.mark.gpu
+
+
+def hea_z0(n, layers):
+    def cd(*p):
+        it = iter(p)
+        for _ in range(layers):
+            for q in range(n):
+                qai.RY(next(it), qubits=[q])
+                qai.RZ(next(it), qubits=[q])
+            for q in range(n - 1):
+                qai.CNOT(qubits=[q, q + 1])
+        return qai.expval(qai.PauliZ(qubits=[0]))
+    return qai.Circuit(cd, n, *([0.1] * (2 * n * layers))).compilecircuit(backend="pytorch")
+
+
+def test_results_alone_equal_results_in_the_mix():
+    rng = np.random.default_rng(11)
+    cache = {}
+
+    def factory(key):
+        if key not in cache:
+            cache[key] = hea_z0(*key)
+        return cache[key]
+
+    reqs = []
+    for i in range(120):
+        n, L = int(rng.integers(4, 15)), int(rng.integers(1, 5))
+        reqs.append(Request(i, (n, L), rng.uniform(0, 2 * np.pi, 2 * n * L).astype(np.float32),
+                            float(i) * 2e-4))
+    eng = ServingEngine(factory, hbm_budget=4 << 30, max_batch=16, n_streams=3, max_inflight=6)
+    eng.run(reqs)
+    assert len(eng.completed) == len(reqs)
+    assert max(eng.batch_sizes) > 1          # batching actually happened
+    stats = latency_stats(reqs)
+    assert stats["p99_ms"] >= stats["p50_ms"] > 0
+    dev = torch.device("cuda", 0)
+    for r in reqs:
+        ex = factory(r.key).executor()
+        row = torch.tensor(r.params[None, :], device=dev)
+        outs, grad = ex.value_and_grad(row, [torch.ones(1, device=dev)])
+        alone_v = outs[0].float().cpu().numpy().reshape(-1)
+        alone_g = grad.cpu().numpy().reshape(-1)
+        assert np.max(np.abs(alone_v - r.value)) <= 2e-6
+        assert np.max(np.abs(alone_g - r.grad)) <= 2e-6
+    for r in reqs[:6]:
+        n, L = r.key
+        t = torch.tensor(r.params.reshape(L, n, 2), dtype=torch.float64, requires_grad=True)
+        ref = O.execute(n, O.hea_gates(n, t)[0], [("expval", ("PauliZ", [0]))])
+        ref.sum().backward()
+        assert abs(float(ref) - float(r.value[0])) <= 1e-5
+        assert np.max(np.abs(t.grad.numpy().reshape(-1) - r.grad)) <= 1e-4
+
+
+def test_budget_limits_inflight_bytes():
+    cache = {}
+
+    def factory(key):
+        if key not in cache:
+            cache[key] = hea_z0(*key)
+        return cache[key]
+    ex = factory((14, 2)).executor()
+    one = ex.memory_bytes(1)
+    eng = ServingEngine(factory, hbm_budget=3 * one, max_batch=64, n_streams=2, max_inflight=8,
+                        graph_max_qubits=0)
+    rng = np.random.default_rng(3)
+    reqs = [Request(i, (14, 2), rng.uniform(0, 6.28, 56).astype(np.float32), 0.0) for i in range(12)]
+    peak = 0
+    for r in reqs:
+        eng.submit(r)
+    while len(eng.completed) < len(reqs):
+        eng.poll()
+        eng.schedule(0.0)
+        peak = max(peak, eng._bytes_in_use)  # pylint: disable=protected-access
+    assert peak <= 3 * one
+
+
+def test_graph_replay_matches_eager_across_replays():
+    """Captured launch sequences: every replay (with eager work and syncs in between) equals eager."""
+    from tedq_amd.serving.engine import GraphedFwdGrad
+    dev = torch.device("cuda", 0)
+    c = hea_z0(10, 3)
+    ex = c.executor()
+    stream = torch.cuda.Stream()
+    runner = GraphedFwdGrad(ex, 4, dev, stream)
+    rng = np.random.default_rng(0)
+    for rep in range(5):
+        params = rng.uniform(0, 6.28, size=(3, 60)).astype(np.float32)
+        ev = runner.run(params)
+        ev.synchronize()
+        got_v, got_g = runner.host_vals[:3].clone(), runner.host_grad[:3].clone()
+        outs, grad = ex.value_and_grad(torch.tensor(params, device=dev), [torch.ones(3, device=dev)])
+        torch.cuda.synchronize()
+        assert np.allclose(got_v.numpy()[:, 0], outs[0].float().cpu().numpy(), atol=1e-6), rep
+        assert np.allclose(got_g.numpy(), grad.cpu().numpy(), atol=1e-6), rep

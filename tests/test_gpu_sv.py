@@ -263,7 +263,7 @@ def test_c2_full_size_properties():
     rows = torch.tensor(theta.reshape(B, -1), dtype=torch.float32, device=DEV, requires_grad=True)
     out = c.execute_batch(rows)
     ex = c.executor()
-    psi, mats, amats = ex.run_forward(rows.detach(), need_grad=True)
+    psi = ex.run_forward(rows.detach())
     norm = (psi.abs() ** 2).double().sum(1)
     assert torch.allclose(norm, torch.ones_like(norm), atol=1e-5)
     ref = O.execute(n, O.hea_gates(n, theta[0])[0], [("expval", ("PauliZ", [q])) for q in range(n)])
@@ -275,7 +275,7 @@ def test_c2_full_size_properties():
     # the reverse sweep returns psi to |0..0>
     lam = ex.seed_adjoint(psi, [torch.ones(B, device=DEV) for _ in range(n)])
     work = psi.clone()
-    ex.run_backward(work, lam, amats, mats)
+    ex.run_backward(rows.detach(), work, lam)
     e0 = torch.zeros_like(work)
     e0[:, 0] = 1
     assert (work - e0).abs().max().item() < 1e-4

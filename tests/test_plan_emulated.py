@@ -149,3 +149,30 @@ def test_hea_plan_two_passes_per_layer():
     assert len(plan.passes) <= 2 * L + 2
     assert all(p.tile_bits[:3] == [0, 1, 2] for p in plan.passes)   # coalesced 64-byte runs
     assert plan.n_gslots == n * L       # RY.RZ fused per qubit and layer
+
+
+def test_two_qubit_unitary_lowering():
+    """4x4 user matrices are lowered exactly (cosine-sine decomposition) and may be inverted."""
+    from scipy.stats import unitary_group
+    n = 4
+    U1 = unitary_group.rvs(4, random_state=1)
+    U2 = unitary_group.rvs(4, random_state=2)
+    u1q = unitary_group.rvs(2, random_state=3)
+
+    def circuit_def():
+        qai.Hadamard(qubits=[0])
+        qai.RY(0.3, qubits=[2])
+        qai.Unitary(U1, qubits=[2, 0])
+        qai.Unitary(U2, qubits=[1, 3]).inverse()
+        qai.Unitary(u1q, qubits=[3])
+        qai.CNOT(qubits=[3, 1])
+        return qai.state()
+    cc = qai.Circuit(circuit_def, n).compilecircuit(backend="pytorch")
+    nq, units, P, pool = lower_circuit(cc)
+    plan = build_plan(nq, units, P, pool, k_max=3, r=2)
+    enc = encode_plan(plan)
+    assert P == 1
+    psi, _, _ = EMU.forward(enc, plan, np.array([0.3]))
+    ref = O.propagate(n, [("Hadamard", [0], []), ("RY", [2], [0.3]), ("Unitary", [2, 0], U1),
+                          ("Unitary", [1, 3], U2.conj().T), ("Unitary", [3], u1q), ("CNOT", [3, 1], [])])
+    assert np.allclose(psi, ref.numpy().reshape(-1), atol=1e-12)
