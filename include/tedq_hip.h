@@ -111,6 +111,44 @@ int tedq_sv_axpy(void* lam, const void* G, int64_t batch, int n_qubits, float sc
 /* out[b][m] = compensated sum over chunks of partial[b][chunk][m]. */
 int tedq_sum_partials(const double* partial, int64_t batch, int64_t nchunk, int M, double* out, void* stream);
 
+/* ---- tensor-network executor ----------------------------------------------- */
+/* desc: 4 + 9*32 int32 — bit counts of the index roles (h kept+shared, m kept
+ * only in A, n kept only in B, k summed), then for each of A_h A_m A_k B_h B_k
+ * B_n C_h C_m C_n (32 slots each) the address bit of every role bit.  Every
+ * index has dimension 2, so this fixes both operand layouts and the result
+ * layout; no permuted copy is made.  Bytes of device workspace the step needs
+ * for its split-k partial sums (0 if none), or -1 for an invalid descriptor. */
+int64_t tedq_tn_workspace_bytes(const int32_t* desc, int64_t batch);
+/* One pairwise contraction step for every batch element:
+ *   C[h,m,n] = sum_k A[h,m,k] B[h,k,n]   (complex64, fp32 accumulation)
+ * a/b/c: dev complex64 with per-batch strides in elements (0 = shared operand);
+ * desc: host copy (validated, sizes the launch) and desc_dev: an identical
+ * device copy read by the kernels.  Replaces one pairwise step of the
+ * reference's contraction tree.contract(arrays, backend='torch')
+ * (pytorch_backend.py:276-354, oe_wrapper.py:57). */
+int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b, int64_t b_bstride, void* c,
+                     int64_t c_bstride, int64_t batch, const int32_t* desc, const int32_t* desc_dev,
+                     void* workspace, int64_t workspace_bytes, void* stream);
+/* Gate tensors of the network's leaves for every parameter row: ket U with
+ * axes (out..., in...) or bra U^dagger, and dV/dtheta 2x2 blocks of trainable
+ * parameters.  recs: dev int32[16] records {code, src0..2, const_off, nq,
+ * kind (0 controlled 2x2 block, 1 constant matrix), bra, off, d0..2, dagger, pad};
+ * leaves: dev complex64 [batch][leaf_stride]; dmats: dev complex64
+ * [batch][dmat_stride].  Replaces the gate operands and their conjugates
+ * (pytorch_backend.py:524-546, 567-1188) for the contraction path. */
+int tedq_tn_gate_tensors(const void* params, int params_are_f64, int64_t batch, int n_params, const void* recs,
+                         int nrec, const void* const_pool, void* leaves, int64_t leaf_stride, void* dmats,
+                         int64_t dmat_stride, int* err_flag, void* stream);
+/* grad[b][p] += upstream[b] * Re(sum E_ket . dU + sum E_bra . dU^dagger) over
+ * the gate of trainable parameter p, from the leaf environments env (same
+ * layout as the leaves).  recs: dev int32[4] {ket_off, bra_off, nq, dslot}.
+ * With the environments from the reverse contraction tree this is the
+ * gradient the reference gets from torch autograd through the contraction
+ * (pytorch_backend.py:276-354). */
+int tedq_tn_param_grad(const void* env, int64_t env_stride, const void* dmats, int64_t dmat_stride,
+                       const void* recs, int n_params, const float* upstream, double* grad, int64_t batch,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -205,6 +205,107 @@ __global__ void k_build_tables(const PT* __restrict__ params, int64_t batch, int
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Tensor-network leaves: full gate tensors for the ket (U, axes (out..., in...))
+// and the bra (U^dagger) of every gate instance, per parameter row, plus the
+// derivative 2x2 blocks dV/dtheta of every trainable parameter.  The reference
+// builds the same operands with torch ops per gate per call
+// (tedq/backends/pytorch_backend.py:524-546 for the conjugated copies).
+// ---------------------------------------------------------------------------
+struct TNGateRec {
+  int code, src0, src1, src2, const_off, nq, kind, bra, off, d0, d1, d2, dagger, pad1, pad2, pad3;
+};
+
+template <typename PT>
+__global__ void k_tn_gate_tensors(const PT* __restrict__ params, int64_t batch, int n_params,
+                                  const TNGateRec* __restrict__ recs, int nrec, const double2* __restrict__ cpool,
+                                  float2* __restrict__ leaves, int64_t leaf_stride, float2* __restrict__ dmats,
+                                  int64_t dmat_stride, int* __restrict__ err) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= batch * nrec) return;
+  const int64_t b = gid / nrec;
+  const TNGateRec r = recs[gid - b * nrec];
+  const PT* prow = params + b * (int64_t)n_params;
+  float2* out = leaves + b * leaf_stride + r.off;
+  const int D = 1 << r.nq;
+  if (r.kind == 1) {  // constant matrix, row-major complex128
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
+        const double2 v = r.bra ? cpool[r.const_off + j * D + i] : cpool[r.const_off + i * D + j];
+        out[i * D + j] = make_float2((float)v.x, (float)(r.bra ? -v.y : v.y));
+      }
+    return;
+  }
+  CompRec cr;
+  cr.code = r.code;
+  cr.dagger = r.dagger;
+  cr.src0 = r.src0;
+  cr.src1 = r.src1;
+  cr.src2 = r.src2;
+  cr.const_off = r.const_off;
+  bool ok = true;
+  const M2 V = comp_eval(cr, prow, cpool, -1, &ok);
+  if (!ok) atomicOr(err, 2);
+  // controlled embedding: identity except the block where every control is 1
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) {
+      const int bi = i - (D - 2), bj = j - (D - 2);
+      cd v;
+      if (bi >= 0 && bj >= 0) {
+        v = r.bra ? cd_conj(V.a[bj * 2 + bi]) : V.a[bi * 2 + bj];
+      } else {
+        v = mkcd(i == j ? 1.0 : 0.0, 0.0);
+      }
+      out[i * D + j] = make_float2((float)v.x, (float)v.y);
+    }
+  const int ds[3] = {r.d0, r.d1, r.d2};
+  for (int w = 0; w < 3; ++w) {
+    if (ds[w] < 0) continue;
+    const M2 dV = comp_eval(cr, prow, cpool, w, &ok);
+    float2* dst = dmats + b * dmat_stride + (int64_t)ds[w] * 4;
+    for (int i = 0; i < 4; ++i) dst[i] = make_float2((float)dV.a[i].x, (float)dV.a[i].y);
+  }
+}
+
+// d value / d theta_p = sum over the gate's ket tensor of E_ket . dU plus over
+// its bra tensor of E_bra . dU^dagger, where E is the environment (the
+// contraction of everything else).  Only the controlled block of dU is
+// non-zero.  grad[b][p] += g[b] * Re(...), accumulated over networks.
+struct TNParamRec {
+  int ket_off, bra_off, nq, dslot;
+};
+
+__global__ void k_tn_param_grad(const float2* __restrict__ env, int64_t env_stride, const float2* __restrict__ dmats,
+                                int64_t dmat_stride, const TNParamRec* __restrict__ recs, int np,
+                                const float* __restrict__ upstream, double* __restrict__ grad, int64_t batch) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= batch * np) return;
+  const int64_t b = gid / np;
+  const int p = (int)(gid - b * np);
+  const TNParamRec r = recs[p];
+  if (r.dslot < 0) return;
+  const float2* dv = dmats + b * dmat_stride + (int64_t)r.dslot * 4;
+  const int D = 1 << r.nq;
+  double acc = 0.0;
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) {
+      const int e = (D - 2 + a) * D + (D - 2 + c);
+      const float2 d = dv[a * 2 + c];
+      if (r.ket_off >= 0) {
+        const float2 x = env[b * env_stride + r.ket_off + e];
+        acc += (double)x.x * d.x - (double)x.y * d.y;
+      }
+      if (r.bra_off >= 0) {
+        // bra tensor entry (a, c) is conj(U[c][a]); its derivative is conj(dV[c][a])
+        const float2 x = env[b * env_stride + r.bra_off + e];
+        const float2 dc = dv[c * 2 + a];
+        acc += (double)x.x * dc.x + (double)x.y * dc.y;
+      }
+    }
+  grad[b * np + p] += (double)upstream[b] * acc;
+}
+
 }  // namespace tedq
 
 using namespace tedq;
@@ -239,4 +340,47 @@ extern "C" int tedq_sv_build_tables(const void* params, int params_are_f64, int6
                        static_cast<float2*>(mats), mat_stride, static_cast<double2*>(amats), err_flag);
   }
   return check_launch("tedq_sv_build_tables");
+}
+
+extern "C" int tedq_tn_gate_tensors(const void* params, int params_are_f64, int64_t batch, int n_params,
+                                    const void* recs, int nrec, const void* const_pool, void* leaves,
+                                    int64_t leaf_stride, void* dmats, int64_t dmat_stride, int* err_flag,
+                                    void* stream) {
+  if (batch <= 0 || nrec < 0 || n_params < 0 || leaves == nullptr || err_flag == nullptr ||
+      (n_params > 0 && params == nullptr)) {
+    set_error("tedq_tn_gate_tensors: bad arguments (batch=%lld records=%d)", (long long)batch, nrec);
+    return kBadArgument;
+  }
+  if (nrec == 0) return kOk;
+  const int threads = 128;
+  const int64_t blocks = (batch * nrec + threads - 1) / threads;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (params_are_f64) {
+    hipLaunchKernelGGL(k_tn_gate_tensors<double>, dim3((unsigned)blocks), dim3(threads), 0, st,
+                       static_cast<const double*>(params), batch, n_params, static_cast<const TNGateRec*>(recs), nrec,
+                       static_cast<const double2*>(const_pool), static_cast<float2*>(leaves), leaf_stride,
+                       static_cast<float2*>(dmats), dmat_stride, err_flag);
+  } else {
+    hipLaunchKernelGGL(k_tn_gate_tensors<float>, dim3((unsigned)blocks), dim3(threads), 0, st,
+                       static_cast<const float*>(params), batch, n_params, static_cast<const TNGateRec*>(recs), nrec,
+                       static_cast<const double2*>(const_pool), static_cast<float2*>(leaves), leaf_stride,
+                       static_cast<float2*>(dmats), dmat_stride, err_flag);
+  }
+  return check_launch("tedq_tn_gate_tensors");
+}
+
+extern "C" int tedq_tn_param_grad(const void* env, int64_t env_stride, const void* dmats, int64_t dmat_stride,
+                                  const void* recs, int n_params, const float* upstream, double* grad, int64_t batch,
+                                  void* stream) {
+  if (env == nullptr || recs == nullptr || upstream == nullptr || grad == nullptr || batch <= 0 || n_params < 0) {
+    set_error("tedq_tn_param_grad: bad arguments");
+    return kBadArgument;
+  }
+  if (n_params == 0) return kOk;
+  const int threads = 128;
+  const int64_t blocks = (batch * n_params + threads - 1) / threads;
+  hipLaunchKernelGGL(k_tn_param_grad, dim3((unsigned)blocks), dim3(threads), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const float2*>(env), env_stride, static_cast<const float2*>(dmats), dmat_stride,
+                     static_cast<const TNParamRec*>(recs), n_params, upstream, grad, batch);
+  return check_launch("tedq_tn_param_grad");
 }

@@ -1,0 +1,371 @@
+// Pairwise tensor-network contraction step on gfx950.
+//
+// Replaces one step of the reference's contraction along its frozen order
+// (tedq/backends/pytorch_backend.py:276-354, `tree.contract(arrays,
+// backend='torch')`, which issues tensordot/einsum = permute + batched complex
+// GEMM per pairwise step).  Here the permutation is never materialised: every
+// index of every tensor has dimension 2, so a tensor's layout is a map from
+// its index bits to address bits, and the kernels gather operands and scatter
+// results through those maps while they contract.
+//
+// One step, per batch element b:
+//     C[h, m, n] = sum_k A[h, m, k] * B[h, k, n]
+// h: indices kept and shared (Hadamard), m: kept, only in A, n: kept, only in
+// B, k: summed.  The caller passes, for each of the nine (tensor, role) pairs,
+// the address bit of every role bit (role bit j of h, m, n or k -> bit
+// position in A, B or C), so any operand and result layout is allowed.
+//
+// Two kernels:
+//   * k_tn_small: a group of LPO lanes per output element strides over k
+//     (lanes adjacent in k, so an inner product over A's low bits reads
+//     contiguous memory), reduced with wave shuffles; for large k the range is
+//     additionally split over S workgroups whose partial sums are added in a
+//     fixed order by k_tn_split_sum (deterministic, no atomics).
+//   * k_tn_gemm: 64x64 output tiles, k in chunks of 16 staged through LDS,
+//     4x4 complex outputs per thread; used when m, n and k are all wide.
+// Arithmetic is complex64 with fp32 accumulation, like the reference's
+// complex64 torch contraction.
+#include "tedq_common.h"
+
+namespace tedq {
+
+constexpr int kMaxRank = 30;  // per-batch tensor of 2^30 complex64 = 8 GiB
+
+// host-side copy of a step descriptor, used to validate it and plan the launch
+struct TNDesc {
+  int nb[4];               // bits of h, m, n, k
+  signed char pos[9][32];  // A_h A_m A_k B_h B_k B_n C_h C_m C_n
+};
+enum { AH = 0, AM, AK, BH, BK, BN, CH, CM, CN };
+
+// Role layout as the kernels hold it in LDS (copied from the device
+// descriptor at kernel start, so that it costs no registers).
+struct TNShared {
+  int nb[4];
+  signed char pos[9][32];
+};
+
+__device__ __forceinline__ void load_desc(TNShared& s, const int32_t* __restrict__ dd) {
+  for (int i = threadIdx.x; i < 4 + 9 * 32; i += blockDim.x) {
+    if (i < 4) s.nb[i] = dd[i];
+    else (&s.pos[0][0])[i - 4] = (signed char)dd[i];
+  }
+  __syncthreads();
+}
+
+// address bits of role value x: bit j of x goes to address bit pos[R][j]
+__device__ __forceinline__ uint32_t scat(const TNShared& s, int R, int nbits, uint32_t x) {
+  uint32_t off = 0;
+  for (int j = 0; j < nbits; ++j) off |= ((x >> j) & 1u) << s.pos[R][j];
+  return off;
+}
+
+constexpr int kTabBits = 10;  // k offsets for the low k bits come from LDS tables
+
+__global__ void __launch_bounds__(256) k_tn_small(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                  float2* __restrict__ C, float2* __restrict__ part,
+                                                  const int32_t* __restrict__ dd, int64_t a_bs, int64_t b_bs,
+                                                  int64_t c_bs, int lpo_log, int split_log, int64_t n_out) {
+  __shared__ uint32_t tka[1 << kTabBits];
+  __shared__ uint32_t tkb[1 << kTabBits];
+  __shared__ TNShared d;
+  load_desc(d, dd);
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  const int tb = nK < kTabBits ? nK : kTabBits;
+  for (int i = threadIdx.x; i < (1 << tb); i += blockDim.x) {
+    tka[i] = scat(d, AK, tb, i);
+    tkb[i] = scat(d, BK, tb, i);
+  }
+  __syncthreads();
+  const int64_t b = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lpo = 1 << lpo_log;
+  const int sub = (int)(g & (lpo - 1));
+  const int64_t rest = g >> lpo_log;
+  const int64_t o = rest >> split_log;          // output element
+  const int s = (int)(rest & ((1 << split_log) - 1));  // k segment
+  const bool valid = o < n_out;
+  const uint32_t oo = valid ? (uint32_t)o : 0u;
+  const uint32_t n = oo & ((1u << nN) - 1);
+  const uint32_t m = (oo >> nN) & ((1u << nM) - 1);
+  const uint32_t h = oo >> (nN + nM);
+  const float2* a = A + b * a_bs + (scat(d, AH, nH, h) | scat(d, AM, nM, m));
+  const float2* bp = B + b * b_bs + (scat(d, BH, nH, h) | scat(d, BN, nN, n));
+  const int64_t kcount = 1ll << nK;
+  const int64_t seg = kcount >> split_log;
+  const int64_t k0 = (int64_t)s * seg;
+  float2 acc = make_float2(0.f, 0.f);
+  if (valid) {
+    const uint32_t lo_mask = (1u << tb) - 1;
+    for (int64_t kb = k0; kb < k0 + seg; kb += (int64_t)lo_mask + 1) {
+      // high k bits are constant over one table span
+      uint32_t ha = 0, hb = 0;
+      if (nK > tb) {
+        const uint32_t hi = (uint32_t)(kb >> tb);
+        for (int j = 0; j < nK - tb; ++j) {
+          ha |= ((hi >> j) & 1u) << d.pos[AK][tb + j];
+          hb |= ((hi >> j) & 1u) << d.pos[BK][tb + j];
+        }
+      }
+      const int64_t span = (seg < (int64_t)lo_mask + 1) ? seg : (int64_t)lo_mask + 1;
+      const uint32_t lo0 = (uint32_t)(kb & lo_mask);
+      for (int64_t i = sub; i < span; i += lpo) {
+        const uint32_t kl = lo0 + (uint32_t)i;
+        const float2 x = a[ha | tka[kl]];
+        const float2 y = bp[hb | tkb[kl]];
+        acc.x = fmaf(x.x, y.x, fmaf(-x.y, y.y, acc.x));
+        acc.y = fmaf(x.x, y.y, fmaf(x.y, y.x, acc.y));
+      }
+    }
+  }
+  for (int off = 1; off < lpo; off <<= 1) {
+    acc.x += __shfl_xor(acc.x, off, 64);
+    acc.y += __shfl_xor(acc.y, off, 64);
+  }
+  if (!valid || sub != 0) return;
+  if (split_log == 0) {
+    C[b * c_bs + (scat(d, CH, nH, h) | scat(d, CM, nM, m) | scat(d, CN, nN, n))] = acc;
+  } else {
+    part[(b << split_log | s) * n_out + o] = acc;
+  }
+}
+
+__global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restrict__ C,
+                               const int32_t* __restrict__ dd, int64_t c_bs, int split_log, int64_t n_out) {
+  __shared__ TNShared d;
+  load_desc(d, dd);
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_out) return;
+  const int64_t b = blockIdx.y;
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2];
+  const uint32_t oo = (uint32_t)o;
+  const uint32_t n = oo & ((1u << nN) - 1);
+  const uint32_t m = (oo >> nN) & ((1u << nM) - 1);
+  const uint32_t h = oo >> (nN + nM);
+  float2 acc = make_float2(0.f, 0.f);
+  const int S = 1 << split_log;
+  for (int s = 0; s < S; ++s) {
+    const float2 v = part[(b * S + s) * n_out + o];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  C[b * c_bs + (scat(d, CH, nH, h) | scat(d, CM, nM, m) | scat(d, CN, nN, n))] = acc;
+}
+
+// 64x64 output tile of one (batch, h); k staged 16 at a time through LDS.
+constexpr int kTile = 64;
+constexpr int kChunk = 16;
+
+__global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                 float2* __restrict__ C, const int32_t* __restrict__ dd,
+                                                 int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles) {
+  __shared__ float2 As[kChunk][kTile + 1];
+  __shared__ float2 Bs[kChunk][kTile + 1];
+  __shared__ uint32_t rowA[kTile], rowC[kTile], colB[kTile], colC[kTile];
+  __shared__ uint32_t ka[kChunk], kb[kChunk];
+  __shared__ TNShared d;
+  load_desc(d, dd);
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  const int t = threadIdx.x;
+  const int64_t b = blockIdx.z;
+  const uint32_t h = blockIdx.y;
+  const int mt = blockIdx.x % m_tiles, nt = blockIdx.x / m_tiles;
+  const uint32_t m0 = (uint32_t)mt * kTile, n0 = (uint32_t)nt * kTile;
+  const uint32_t msz = 1u << nM, nsz = 1u << nN;
+  if (t < kTile) {
+    const uint32_t m = m0 + t;
+    rowA[t] = scat(d, AH, nH, h) | scat(d, AM, nM, m < msz ? m : 0);
+    rowC[t] = scat(d, CH, nH, h) | scat(d, CM, nM, m < msz ? m : 0);
+  } else if (t < 2 * kTile) {
+    const uint32_t n = n0 + (t - kTile);
+    colB[t - kTile] = scat(d, BH, nH, h) | scat(d, BN, nN, n < nsz ? n : 0);
+    colC[t - kTile] = scat(d, CN, nN, n < nsz ? n : 0);
+  }
+  const float2* a = A + b * a_bs;
+  const float2* bp = B + b * b_bs;
+  const int ty = t >> 4, tx = t & 15;
+  float2 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = make_float2(0.f, 0.f);
+  const uint32_t ksz = 1u << nK;
+  for (uint32_t k0 = 0; k0 < ksz; k0 += kChunk) {
+    if (t >= 2 * kTile && t < 2 * kTile + kChunk) {
+      const uint32_t k = k0 + (t - 2 * kTile);
+      ka[t - 2 * kTile] = scat(d, AK, nK, k < ksz ? k : 0);
+    } else if (t >= 2 * kTile + kChunk && t < 2 * kTile + 2 * kChunk) {
+      const uint32_t k = k0 + (t - 2 * kTile - kChunk);
+      kb[t - 2 * kTile - kChunk] = scat(d, BK, nK, k < ksz ? k : 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = t + 256 * r;
+      const int mi = e & (kTile - 1), kk = e >> 6;
+      const bool ok = (m0 + mi < msz) && (k0 + kk < ksz);
+      As[kk][mi] = ok ? a[rowA[mi] | ka[kk]] : make_float2(0.f, 0.f);
+      const bool okb = (n0 + mi < nsz) && (k0 + kk < ksz);
+      Bs[kk][mi] = okb ? bp[colB[mi] | kb[kk]] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kChunk; ++kk) {
+      float2 x[4], y[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) y[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc[i][j].x = fmaf(x[i].x, y[j].x, fmaf(-x[i].y, y[j].y, acc[i][j].x));
+          acc[i][j].y = fmaf(x[i].x, y[j].y, fmaf(x[i].y, y[j].x, acc[i][j].y));
+        }
+    }
+    __syncthreads();
+  }
+  float2* c = C + b * c_bs;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t mi = ty * 4 + i;
+    if (m0 + mi >= msz) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t ni = tx * 4 + j;
+      if (n0 + ni < nsz) c[rowC[mi] | colC[ni]] = acc[i][j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch planning shared by the workspace query and the launch
+// ---------------------------------------------------------------------------
+struct TNLaunch {
+  bool gemm;
+  int lpo_log, split_log;
+  int64_t n_out;
+};
+
+static int parse_desc(const int32_t* desc, TNDesc* d) {
+  for (int r = 0; r < 4; ++r) {
+    if (desc[r] < 0 || desc[r] > kMaxRank) return -1;
+    d->nb[r] = desc[r];
+  }
+  const int nH = d->nb[0], nM = d->nb[1], nN = d->nb[2], nK = d->nb[3];
+  const int roles[9] = {nH, nM, nK, nH, nK, nN, nH, nM, nN};
+  const int rankA = nH + nM + nK, rankB = nH + nK + nN, rankC = nH + nM + nN;
+  if (rankA > kMaxRank || rankB > kMaxRank || rankC > kMaxRank) return -1;
+  const int ranks[3] = {rankA, rankB, rankC};
+  for (int t = 0; t < 3; ++t) {
+    uint32_t seen = 0;
+    for (int r = 0; r < 3; ++r) {
+      const int q = 3 * t + r;
+      for (int j = 0; j < 32; ++j) {
+        const int p = desc[4 + q * 32 + j];
+        if (j < roles[q]) {
+          if (p < 0 || p >= ranks[t] || (seen >> p) & 1u) return -1;
+          seen |= 1u << p;
+          d->pos[q][j] = (signed char)p;
+        } else {
+          d->pos[q][j] = 0;
+        }
+      }
+    }
+    if (ranks[t] < 32 && seen != ((1u << ranks[t]) - 1)) return -1;
+  }
+  return 0;
+}
+
+static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
+  TNLaunch L;
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  L.n_out = 1ll << (nH + nM + nN);
+  L.gemm = nM >= 6 && nN >= 6 && nK >= 4;
+  L.lpo_log = 0;
+  L.split_log = 0;
+  if (L.gemm) return L;
+  // aim for >= 2^19 threads in flight and <= 2^12 k iterations per thread
+  const int64_t want = 1ll << 19;
+  int64_t threads = L.n_out * batch;
+  while (L.lpo_log < 6 && L.lpo_log < nK && threads < want) {
+    ++L.lpo_log;
+    threads *= 2;
+  }
+  while (L.split_log < nK - L.lpo_log - kTabBits && (threads < want || nK - L.lpo_log - L.split_log > 12)) {
+    ++L.split_log;
+    threads *= 2;
+  }
+  return L;
+}
+
+}  // namespace tedq
+
+using namespace tedq;
+
+// desc: 4 + 9*32 int32 = bit counts of (h, m, n, k), then the address bit of
+// every role bit for A_h A_m A_k B_h B_k B_n C_h C_m C_n (32 slots per role).
+// tedq_tn_contract takes it twice: in host memory (validated, used to size
+// the launch) and as an identical copy in device memory (read by the kernels).
+extern "C" int64_t tedq_tn_workspace_bytes(const int32_t* desc, int64_t batch) {
+  TNDesc d;
+  if (desc == nullptr || batch <= 0 || parse_desc(desc, &d) != 0) return -1;
+  const TNLaunch L = plan_launch(d, batch);
+  if (L.split_log == 0) return 0;
+  return (int64_t)sizeof(float2) * batch * (L.n_out << L.split_log);
+}
+
+extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b, int64_t b_bstride, void* c,
+                                int64_t c_bstride, int64_t batch, const int32_t* desc, const int32_t* desc_dev,
+                                void* workspace, int64_t workspace_bytes, void* stream) {
+  TNDesc d;
+  if (a == nullptr || b == nullptr || c == nullptr || desc == nullptr || desc_dev == nullptr || batch <= 0 ||
+      batch > 65535) {
+    set_error("tedq_tn_contract: bad arguments (batch=%lld)", (long long)batch);
+    return kBadArgument;
+  }
+  if (parse_desc(desc, &d) != 0) {
+    set_error("tedq_tn_contract: inconsistent index layout (ranks above %d or bits not a permutation)", kMaxRank);
+    return kBadArgument;
+  }
+  const TNLaunch L = plan_launch(d, batch);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float2* A = static_cast<const float2*>(a);
+  const float2* Bp = static_cast<const float2*>(b);
+  float2* Cp = static_cast<float2*>(c);
+  if (L.gemm) {
+    const int m_tiles = (int)(((1ll << d.nb[1]) + kTile - 1) / kTile);
+    const int64_t n_tiles = ((1ll << d.nb[2]) + kTile - 1) / kTile;
+    const int64_t gx = m_tiles * n_tiles;
+    const int64_t gy = 1ll << d.nb[0];
+    if (gx > 0x7fffffffll || gy > 65535) {
+      set_error("tedq_tn_contract: grid too large (%lld x %lld)", (long long)gx, (long long)gy);
+      return kBadArgument;
+    }
+    hipLaunchKernelGGL(k_tn_gemm, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp, Cp, desc_dev,
+                       a_bstride, b_bstride, c_bstride, m_tiles);
+    return check_launch("tedq_tn_contract(gemm)");
+  }
+  const int64_t need = L.split_log ? (int64_t)sizeof(float2) * batch * (L.n_out << L.split_log) : 0;
+  if (need > 0 && (workspace == nullptr || workspace_bytes < need)) {
+    set_error("tedq_tn_contract: workspace of %lld bytes required, %lld given", (long long)need,
+              (long long)workspace_bytes);
+    return kBadArgument;
+  }
+  const int64_t threads = L.n_out << (L.lpo_log + L.split_log);
+  const int64_t blocks = (threads + 255) / 256;
+  if (blocks > 0x7fffffffll) {
+    set_error("tedq_tn_contract: grid too large (%lld blocks)", (long long)blocks);
+    return kBadArgument;
+  }
+  float2* part = static_cast<float2*>(workspace);
+  hipLaunchKernelGGL(k_tn_small, dim3((unsigned)blocks, (unsigned)batch), dim3(256), 0, st, A, Bp, Cp, part, desc_dev,
+                     a_bstride, b_bstride, c_bstride, L.lpo_log, L.split_log, L.n_out);
+  int rc = check_launch("tedq_tn_contract(small)");
+  if (rc != kOk || L.split_log == 0) return rc;
+  const int64_t sb = (L.n_out + 255) / 256;
+  hipLaunchKernelGGL(k_tn_split_sum, dim3((unsigned)sb, (unsigned)batch), dim3(256), 0, st, part, Cp, desc_dev,
+                     c_bstride, L.split_log, L.n_out);
+  return check_launch("tedq_tn_contract(split sum)");
+}

@@ -59,7 +59,11 @@ class _AdjointFunction(torch.autograd.Function):
         psi = executor.run_forward(rows)
         outs = executor.measure(psi)
         ctx.executor = executor
-        ctx.save_for_backward(rows, psi)
+        if isinstance(psi, torch.Tensor):
+            ctx.save_for_backward(rows, psi)
+        else:                               # contraction mode: a run record, recomputed in backward
+            ctx.save_for_backward(rows)
+            ctx.run = psi
         res = []
         for spec, o in zip(executor.specs, outs):
             if spec.kind in ("zexp", "fexp"):
@@ -72,8 +76,12 @@ class _AdjointFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *grads):  # pylint: disable=arguments-differ
-        rows, psi = ctx.saved_tensors
         ex = ctx.executor
+        if hasattr(ctx, "run"):
+            (rows,) = ctx.saved_tensors
+            grad = ex.run_backward(rows, ctx.run, ex.seed_adjoint(ctx.run, grads), grad_dtype=rows.dtype)
+            return None, grad
+        rows, psi = ctx.saved_tensors
         lam = ex.seed_adjoint(psi, grads)
         grad = ex.run_backward(rows, psi.clone(), lam, grad_dtype=rows.dtype)
         return None, grad

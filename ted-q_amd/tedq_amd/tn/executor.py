@@ -1,0 +1,468 @@
+"""Tensor-network execution of compiled circuits on the GPU.
+
+Mirrors the reference's contraction mode (tedq/backends/pytorch_backend.py:
+242-355): one bra/ket network per measurement (``gen_tensor_networks``), the
+zero vectors, gate operands, observable and conjugated gate operands as
+leaves, contracted pairwise along a path, then ``squeeze(result.real)`` for
+expectation values and probabilities.
+
+What is this module's own:
+
+* leaves are built on the device for every parameter row at once (one
+  launch: tedq_tn_gate_tensors), with the derivative blocks of trainable
+  parameters;
+* every network is first reduced by cancelling ket/bra gate pairs outside
+  the observable's light cone (``cancel_inverse_pairs``), then given a
+  deterministic path (``search_path``); path and index layouts are frozen at
+  construction;
+* each pairwise step is one tedq_tn_contract launch over the whole parameter
+  batch, reading operands and writing the result through index-bit layouts
+  (no permuted copies);
+* gradients come from the reverse contraction tree: the environment of a
+  step's result gives the environments of its two operands (two more
+  contraction launches), and the environments of the gate leaves give
+  d value / d theta (tedq_tn_param_grad).  Intermediates are recomputed per
+  network in the backward pass, and when they do not fit the memory cap the
+  path is cut into segments whose boundary tensors are checkpointed and whose
+  insides are recomputed.
+"""
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ..engine import hip
+from ..engine.sv import MeasSpec
+from ..measurement import Expectation, Probability, State
+from .network import cancel_inverse_pairs
+from .path import search_path
+
+_BLOCK_CODE = {"RX": 14, "RY": 15, "RZ": 16, "Rot": 17, "PhaseShift": 18,
+               "CRX": 14, "CRY": 15, "CRZ": 16, "ControlledPhaseShift": 18}
+_NO_SRC = -2 ** 31
+_DESC_WORDS = 4 + 9 * 32
+_ROLE = {"AH": 0, "AM": 1, "AK": 2, "BH": 3, "BK": 4, "BN": 5, "CH": 6, "CM": 7, "CN": 8}
+
+
+def _bitpos(lst):
+    r = len(lst)
+    return {s: r - 1 - i for i, s in enumerate(lst)}
+
+
+def make_desc(la, lb, lc):
+    """Descriptor of C = contract(A, B) for index lists (most significant first) of A, B and C."""
+    sa, sb, sc = set(la), set(lb), set(lc)
+    h = sa & sb & sc
+    m = (sa - sb) & sc
+    n = (sb - sa) & sc
+    k = (sa & sb) - sc
+    if (sa | sb) - sc - k or sc - (sa | sb):
+        raise ValueError("contraction step: an index is neither kept nor summed over both operands")
+    pa, pb, pc = _bitpos(la), _bitpos(lb), _bitpos(lc)
+    roles = {
+        "h": sorted(h, key=pc.get), "m": sorted(m, key=pc.get),
+        "n": sorted(n, key=pc.get), "k": sorted(k, key=pa.get),
+    }
+    d = np.zeros(_DESC_WORDS, dtype=np.int32)
+    d[0:4] = [len(roles["h"]), len(roles["m"]), len(roles["n"]), len(roles["k"])]
+
+    def put(role_name, syms, pos):
+        base = 4 + 32 * _ROLE[role_name]
+        for j, s in enumerate(syms):
+            d[base + j] = pos[s]
+
+    put("AH", roles["h"], pa)
+    put("AM", roles["m"], pa)
+    put("AK", roles["k"], pa)
+    put("BH", roles["h"], pb)
+    put("BK", roles["k"], pb)
+    put("BN", roles["n"], pb)
+    put("CH", roles["h"], pc)
+    put("CM", roles["m"], pc)
+    put("CN", roles["n"], pc)
+    return d
+
+
+@dataclass
+class _Step:
+    a: int
+    b: int
+    c: int
+    fwd: int            # row of the descriptor table
+    bwd_a: int          # descriptor of E_A = contract(E_C, B), -1 if A needs no environment
+    bwd_b: int
+
+
+class _Network:
+    """One reduced network with its frozen path and index layouts."""
+
+    def __init__(self, tn, leaf_src, desc_rows, trials):
+        red = cancel_inverse_pairs(tn)
+        inputs = [list(t) for t in red.input_indices]
+        sources = list(red.sources)
+        if len(inputs) < 2:                       # keep at least one step
+            inputs.append([])
+            sources.append(("one",))
+        self.measurement = tn.measurement
+        self.output = list(tn.output_indices)
+        self.n_inputs = len(inputs)
+        self.sources = sources
+        self.leaves = [leaf_src(s) for s in sources]
+        path, sets, info = search_path(inputs, self.output, trials)
+        self.path_info = info
+        self.path = path
+        lay = [list(t) for t in inputs]
+        # which leaves and intermediates need an environment (gate leaves and everything above them)
+        needs = [s[0] in ("gate", "adj") for s in sources]
+        self.steps = []
+        for s, (i, j) in enumerate(path):
+            res = sets[self.n_inputs + s]
+            if s == len(path) - 1:
+                out = list(self.output)
+                if set(out) != set(res):
+                    raise ValueError("tensor network: final tensor does not carry the output indices")
+            else:
+                la, lb = lay[i], lay[j]
+                out = [x for x in la if x in res] + [x for x in lb if x in res and x not in la]
+            lay.append(out)
+            needs.append(needs[i] or needs[j])
+            fwd = desc_rows(make_desc(lay[i], lay[j], out))
+            ba = desc_rows(make_desc(out, lay[j], lay[i])) if needs[i] else -1
+            bb = desc_rows(make_desc(lay[i], out, lay[j])) if needs[j] else -1
+            self.steps.append(_Step(i, j, self.n_inputs + s, fwd, ba, bb))
+        self.lay = lay
+        self.ranks = [len(x) for x in lay]
+        self.needs = needs
+        # step index of the consumer of every tensor id
+        self.consumer = {}
+        for s, st in enumerate(self.steps):
+            self.consumer[st.a] = s
+            self.consumer[st.b] = s
+        self.root = self.n_inputs + len(self.steps) - 1
+
+    def intermediate_elems(self):
+        return sum(1 << self.ranks[st.c] for st in self.steps)
+
+
+class TNExecutor:
+    """Contraction-mode executor of one compiled circuit (all its measurements).
+
+    Same duck interface as the state-vector executor: ``run_forward``,
+    ``measure``, ``seed_adjoint``, ``run_backward``, ``value_and_grad``,
+    ``specs`` and ``n_params``.
+    """
+
+    def __init__(self, cc, trials=None, max_bytes=64 << 30):
+        from .path import SEARCH_TRIALS  # pylint: disable=import-outside-toplevel
+        trials = SEARCH_TRIALS if trials is None else trials
+        self.n = cc._num_qubits  # pylint: disable=protected-access
+        self.max_bytes = max_bytes
+        pool = []          # complex128 constants: matrices and constant parameter values
+        recs = []
+        self._gate_off = {}
+        off = 0
+        flat = 0
+        param_recs = []
+        for idx, op in cc._operators.items():  # pylint: disable=protected-access
+            trainable = cc._trainableparams[idx]  # pylint: disable=protected-access
+            nq = len(op.qubits)
+            D = 1 << nq
+            srcs = [_NO_SRC] * 3
+            dsl = [-1] * 3
+            for i, pos in enumerate(trainable):
+                srcs[pos] = flat + i
+                dsl[pos] = flat + i
+            if trainable:
+                name = op.name[:-4] if getattr(op, "is_inverse", False) else op.name
+                if name not in _BLOCK_CODE:
+                    raise NotImplementedError(f"trainable {op.name} is not supported in contraction mode")
+                for pos in range(op.num_params):
+                    if srcs[pos] == _NO_SRC:
+                        from ..operators import as_float  # pylint: disable=import-outside-toplevel
+                        pool.append(complex(as_float(cc._operatorparams[idx][pos]), 0.0))  # pylint: disable=protected-access
+                        srcs[pos] = -1 - (len(pool) - 1)
+                code, kind, coff = _BLOCK_CODE[name], 0, 0
+            else:
+                mat = np.asarray(op.matrix, dtype=np.complex128).reshape(D, D)
+                coff = len(pool)
+                pool.extend(mat.reshape(-1).tolist())
+                code, kind = 0, 1
+            dag = int(bool(getattr(op, "is_inverse", False)))
+            ket, bra = off, off + D * D
+            off += 2 * D * D
+            self._gate_off[idx] = (ket, bra)
+            for is_bra, o in ((0, ket), (1, bra)):
+                d = dsl if not is_bra else [-1, -1, -1]
+                recs.append([code, srcs[0], srcs[1], srcs[2], coff, nq, kind, is_bra, o, d[0], d[1], d[2],
+                             dag, 0, 0, 0])
+            for i, _ in enumerate(trainable):
+                param_recs.append([ket, bra, nq, flat + i])
+            flat += len(trainable)
+        self.n_params = flat
+        self.leaf_stride = max(off, 1)
+        self._recs = np.asarray(recs, dtype=np.int32).reshape(-1, 16)
+        self._param_recs = np.asarray(param_recs, dtype=np.int32).reshape(-1, 4)
+        self._pool = np.asarray(pool if pool else [0j], dtype=np.complex128)
+
+        # constant leaves (batch stride 0): |0>, the scalar 1, observables
+        consts = [np.array([1.0, 0.0], dtype=np.complex64), np.array([1.0], dtype=np.complex64)]
+        const_off = [0, 2]
+        self._const_len = 3
+        descs = []
+
+        def desc_rows(d):
+            descs.append(d)
+            return len(descs) - 1
+
+        self.networks = []
+        self.specs = []
+        for tn in cc._tensor_networks:  # pylint: disable=protected-access
+            meas = tn.measurement
+
+            def leaf_src(s, meas=meas):
+                kind = s[0]
+                if kind == "gate":
+                    return ("arena", self._gate_off[s[1]][0])
+                if kind == "adj":
+                    return ("arena", self._gate_off[s[1]][1])
+                if kind == "zero":
+                    return ("const", 0)
+                if kind == "one":
+                    return ("const", 2)
+                mat = meas.obs[s[1]].matrix if kind == "obs" else meas.obs.matrix
+                m = np.asarray(mat, dtype=np.complex64).reshape(-1)
+                o = self._const_len
+                consts.append(m)
+                const_off.append(o)
+                self._const_len += m.size
+                return ("const", o)
+
+            net = _Network(tn, leaf_src, desc_rows, trials)
+            self.networks.append(net)
+            if meas.return_type is Expectation:
+                self.specs.append(MeasSpec("fexp", ()))
+            elif meas.return_type is Probability:
+                k = len(net.output)
+                self.specs.append(MeasSpec("probs", tuple([2] * k)))
+            elif meas.return_type is State:
+                self.specs.append(MeasSpec("state", tuple([2] * self.n)))
+            else:
+                raise NotImplementedError(f"measurement {meas.return_type} in contraction mode")
+        self._consts = np.concatenate(consts).astype(np.complex64)
+        self._descs = np.stack(descs).astype(np.int32) if descs else np.zeros((1, _DESC_WORDS), np.int32)
+        self._dev = {}
+
+    # ------------------------------------------------------------------
+    def _buffers(self, device):
+        key = str(device)
+        if key not in self._dev:
+            self._dev[key] = {
+                "recs": torch.from_numpy(self._recs.reshape(-1).copy()).to(device),
+                "precs": torch.from_numpy(self._param_recs.reshape(-1).copy()).to(device),
+                "pool": torch.from_numpy(self._pool.view(np.float64).copy()).to(device),
+                "consts": torch.from_numpy(self._consts.copy()).to(device),
+                "descs": torch.from_numpy(self._descs.reshape(-1).copy()).to(device),
+                "err": torch.zeros(1, dtype=torch.int32, device=device),
+            }
+        return self._dev[key]
+
+    def _leaves(self, rows):
+        dev = rows.device
+        B = rows.shape[0]
+        buf = self._buffers(dev)
+        leaves = torch.empty((B, self.leaf_stride), dtype=torch.complex64, device=dev)
+        dmats = torch.empty((B, max(self.n_params, 1) * 4), dtype=torch.complex64, device=dev)
+        hip.call("tedq_tn_gate_tensors", hip.ptr(rows), int(rows.dtype == torch.float64), B, self.n_params,
+                 hip.ptr(buf["recs"]), self._recs.shape[0], hip.ptr(buf["pool"]), hip.ptr(leaves),
+                 self.leaf_stride, hip.ptr(dmats), dmats.shape[1], hip.ptr(buf["err"]), hip.stream_handle(dev))
+        return leaves, dmats
+
+    def _operand(self, net, vals, tid, leaves, buf):
+        """(tensor, batch stride, element offset) of tensor id ``tid``."""
+        if tid < net.n_inputs:
+            where, off = net.leaves[tid]
+            if where == "arena":
+                return leaves, self.leaf_stride, off
+            return buf["consts"], 0, off
+        t = vals[tid]
+        return t, t.shape[1], 0
+
+    def _contract(self, row, a, b, c, B, dev, buf):
+        (ta, sa, oa), (tb, sb, ob), (tc, sc, oc) = a, b, c
+        lib = hip.load()
+        desc = self._descs[row]
+        hdesc = desc.ctypes.data_as(hip.c_vp)
+        ws_bytes = lib.tedq_tn_workspace_bytes(hdesc, B)
+        if ws_bytes < 0:
+            raise hip.EngineError("tensor network: invalid step descriptor")
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev) if ws_bytes else None
+        ddesc = ctypes.c_void_p(buf["descs"].data_ptr() + row * _DESC_WORDS * 4)
+        hip.call("tedq_tn_contract", ctypes.c_void_p(ta.data_ptr() + 8 * oa), sa,
+                 ctypes.c_void_p(tb.data_ptr() + 8 * ob), sb, ctypes.c_void_p(tc.data_ptr() + 8 * oc), sc, B,
+                 hdesc, ddesc, hip.ptr(ws), ws_bytes, hip.stream_handle(dev))
+        return ws
+
+    def _forward_network(self, net, leaves, B, dev, keep=None):
+        """Contract one network; ``keep(tid)`` says which intermediates to retain.  Returns (root, kept)."""
+        buf = self._buffers(dev)
+        vals = {}
+        kept = {}
+        for st in net.steps:
+            out = torch.empty((B, 1 << net.ranks[st.c]), dtype=torch.complex64, device=dev)
+            vals[st.c] = out
+            self._contract(st.fwd, self._operand(net, vals, st.a, leaves, buf),
+                           self._operand(net, vals, st.b, leaves, buf), (out, out.shape[1], 0), B, dev, buf)
+            for t in (st.a, st.b):
+                if t >= net.n_inputs:
+                    if keep is not None and keep(t):
+                        kept[t] = vals[t]
+                    del vals[t]
+        root = vals[net.root]
+        return root, kept
+
+    # ------------------------------------------------------------------
+    def run_forward(self, rows, need_grad=False):  # pylint: disable=unused-argument
+        """Contract every network for [B, P] rows.  Returns a run record for ``measure``/``run_backward``."""
+        B = rows.shape[0]
+        dev = rows.device
+        leaves, dmats = self._leaves(rows)
+        roots = [self._forward_network(net, leaves, B, dev)[0] for net in self.networks]
+        return {"rows": rows, "leaves": leaves, "dmats": dmats, "roots": roots}
+
+    def measure(self, run):
+        outs = []
+        B = run["rows"].shape[0]
+        for spec, root in zip(self.specs, run["roots"]):
+            if spec.kind == "fexp":
+                outs.append(root.reshape(B).real)
+            elif spec.kind == "probs":
+                outs.append(root.real.reshape((B,) + spec.shape))
+            else:
+                outs.append(root.reshape((B,) + spec.shape))
+        return outs
+
+    def seed_adjoint(self, run, grads):
+        """Upstream gradients per measurement (kept with the run; None = no contribution)."""
+        return list(grads)
+
+    # ------------------------------------------------------------------
+    def _segments(self, net, B):
+        total = net.intermediate_elems() * B * 8
+        if total <= self.max_bytes:
+            return [list(range(len(net.steps)))]
+        seg = max(1, int(math.ceil(math.sqrt(len(net.steps)))))
+        return [list(range(s, min(s + seg, len(net.steps)))) for s in range(0, len(net.steps), seg)]
+
+    def _backward_network(self, net, leaves, dmats, root_env, grad, B, dev):
+        buf = self._buffers(dev)
+        segs = self._segments(net, B)
+        seg_of = {}
+        for si, seg in enumerate(segs):
+            for s in seg:
+                seg_of[net.steps[s].c] = si
+        # forward, checkpointing the tensors that cross a segment boundary
+        if len(segs) == 1:
+            def keep(_t):
+                return True
+        else:
+            def keep(t):
+                return seg_of.get(t, -1) != seg_of.get(net.steps[net.consumer[t]].c, -1)
+        root, stored = self._forward_network(net, leaves, B, dev, keep)
+        env_arena = torch.zeros((B, self.leaf_stride), dtype=torch.complex64, device=dev)
+        envs = {net.root: root_env}
+        for si in range(len(segs) - 1, -1, -1):
+            seg = segs[si]
+            vals = dict(stored)
+            if len(segs) > 1:
+                # recompute the inside of this segment from its checkpointed inputs
+                for s in seg:
+                    st = net.steps[s]
+                    if st.c in vals:
+                        continue
+                    out = torch.empty((B, 1 << net.ranks[st.c]), dtype=torch.complex64, device=dev)
+                    vals[st.c] = out
+                    self._contract(st.fwd, self._operand(net, vals, st.a, leaves, buf),
+                                   self._operand(net, vals, st.b, leaves, buf), (out, out.shape[1], 0), B, dev,
+                                   buf)
+            for s in reversed(seg):
+                st = net.steps[s]
+                ec = envs.pop(st.c, None)
+                if ec is None:
+                    continue
+                ecop = (ec, ec.shape[1], 0)
+                for which, desc_row, tid in (("a", st.bwd_a, st.a), ("b", st.bwd_b, st.b)):
+                    if desc_row < 0:
+                        continue
+                    if tid < net.n_inputs:
+                        where, off = net.leaves[tid]
+                        if where != "arena":
+                            continue
+                        dst = (env_arena, self.leaf_stride, off)
+                    else:
+                        e = torch.empty((B, 1 << net.ranks[tid]), dtype=torch.complex64, device=dev)
+                        envs[tid] = e
+                        dst = (e, e.shape[1], 0)
+                    if which == "a":
+                        self._contract(desc_row, ecop, self._operand(net, vals, st.b, leaves, buf), dst, B, dev, buf)
+                    else:
+                        self._contract(desc_row, self._operand(net, vals, st.a, leaves, buf), ecop, dst, B, dev, buf)
+                if st.c in vals and st.c != net.root:
+                    del vals[st.c]
+            for s in seg:
+                stored.pop(net.steps[s].c, None)
+        if self.n_params:
+            self._param_grad(env_arena, dmats, grad, B)
+        return root
+
+    def _param_grad(self, env_arena, dmats, grad, B):
+        dev = env_arena.device
+        buf = self._buffers(dev)
+        ones = torch.ones(B, dtype=torch.float32, device=dev)
+        hip.call("tedq_tn_param_grad", hip.ptr(env_arena), self.leaf_stride, hip.ptr(dmats), dmats.shape[1],
+                 hip.ptr(buf["precs"]), self.n_params, hip.ptr(ones), hip.ptr(grad), B, hip.stream_handle(dev))
+
+    @staticmethod
+    def _root_env(spec, g, B, net):
+        """Environment of the root: conj of the upstream gradient (real outputs: the gradient itself)."""
+        if spec.kind == "fexp":
+            return g.to(torch.float32).reshape(B, 1).to(torch.complex64)
+        if spec.kind == "probs":
+            return g.to(torch.float32).reshape(B, 1 << len(net.output)).to(torch.complex64)
+        return g.to(torch.complex64).reshape(B, -1).conj().contiguous()
+
+    def run_backward(self, rows, run, grads, grad_dtype=torch.float32):
+        """Gradient [B, P] of sum_m <grads_m, out_m> by the reverse contraction tree of every network."""
+        B = rows.shape[0]
+        dev = rows.device
+        grad = torch.zeros((B, max(self.n_params, 1)), dtype=torch.float64, device=dev)
+        for spec, net, g in zip(self.specs, self.networks, grads):
+            if g is None:
+                continue
+            self._backward_network(net, run["leaves"], run["dmats"], self._root_env(spec, g, B, net), grad, B, dev)
+        return grad[:, :self.n_params].to(grad_dtype)
+
+    def value_and_grad(self, rows, grads_out):
+        """Outputs and gradient, contracting each network once with its intermediates kept for the backward."""
+        B = rows.shape[0]
+        dev = rows.device
+        leaves, dmats = self._leaves(rows)
+        grad = torch.zeros((B, max(self.n_params, 1)), dtype=torch.float64, device=dev)
+        roots = []
+        for spec, net, g in zip(self.specs, self.networks, grads_out):
+            if g is None:
+                roots.append(self._forward_network(net, leaves, B, dev)[0])
+            else:
+                roots.append(self._backward_network(net, leaves, dmats, self._root_env(spec, g, B, net), grad, B,
+                                                    dev))
+        outs = self.measure({"rows": rows, "roots": roots})
+        return outs, grad[:, :self.n_params].to(rows.dtype)
+
+    def memory_bytes(self, batch, with_grad=True):  # pylint: disable=unused-argument
+        peak = max((net.intermediate_elems() for net in self.networks), default=0)
+        return batch * 8 * (self.leaf_stride * 2 + min(peak, self.max_bytes // max(batch * 8, 1)))
+
+    def path_report(self):
+        return [dict(net.path_info, steps=len(net.steps), tensors=net.n_inputs) for net in self.networks]

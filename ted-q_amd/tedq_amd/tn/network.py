@@ -115,3 +115,57 @@ def gen_tensor_networks(num_qubits, operators, appliedqubits, measurements):
             size[s] = 2
         networks.append(TensorNetwork(ix, out, size, src, meas))
     return networks
+
+
+def cancel_inverse_pairs(tn):
+    """Drop ket/bra gate pairs that multiply to the identity (the light-cone rule).
+
+    A ket gate ``U`` with indices [new..., old...] and its bra copy
+    ``U^dagger`` with indices [new'..., old'...] contract to the identity
+    when old' == new (the bra copy consumes exactly the ket gate's outputs,
+    in order): both are removed and new' is renamed to old.  Processed in the
+    bra's order (outermost pair first), so cancellations cascade inward from
+    the closing <0| vectors.  A <0| vector and a |0> vector left on the same
+    index contract to 1 and are removed too.  Pairs that touch an output
+    index (probability outputs) are kept.  The reference contracts these
+    pairs numerically (its TensorNetwork.simplify is off in every benchmark,
+    tedq/tensor_network/tensor_network.py:90-221); the value is the same up to
+    single-precision rounding.
+    """
+    ix = [list(t) for t in tn.input_indices]
+    src = list(tn.sources)
+    out = set(tn.output_indices)
+    alive = [True] * len(ix)
+    ket = {s[1]: t for t, s in enumerate(src) if s[0] == "gate"}
+    ren = {}
+
+    def root(s):
+        while s in ren:
+            s = ren[s]
+        return s
+
+    for t, s in enumerate(src):
+        if s[0] != "adj" or s[1] not in ket:
+            continue
+        g = ket[s[1]]
+        k = len(ix[g]) // 2
+        knew = [root(x) for x in ix[g][:k]]
+        bold = [root(x) for x in ix[t][k:]]
+        if knew == bold and not out.intersection(knew):
+            for a, b in zip(ix[t][:k], ix[g][k:]):
+                ren[a] = root(b)
+            alive[g] = alive[t] = False
+    kept = [t for t in range(len(ix)) if alive[t]]
+    new_ix = [[root(x) for x in ix[t]] for t in kept]
+    new_src = [src[t] for t in kept]
+    zeros = {}
+    for j, (t, s) in enumerate(zip(new_ix, new_src)):
+        if s[0] == "zero" and len(t) == 1 and t[0] not in out:
+            zeros.setdefault(t[0], []).append(j)
+    drop = {j for js in zeros.values() if len(js) == 2 for j in js}
+    fin_ix = [t for j, t in enumerate(new_ix) if j not in drop]
+    fin_src = [s for j, s in enumerate(new_src) if j not in drop]
+    size = {x: 2 for t in fin_ix for x in t}
+    for x in tn.output_indices:
+        size[x] = 2
+    return TensorNetwork(fin_ix, list(tn.output_indices), size, fin_src, tn.measurement)

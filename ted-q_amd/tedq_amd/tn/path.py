@@ -20,22 +20,34 @@ The result is an SSA path: step s contracts ids (i, j) and creates id
 """
 
 import heapq
+import math
+import random
 
 
-def _cost(a, b, keep_counts, out_set):
+def _cost(a, b, keep_counts, out_set, alpha=1.0):
     union = a | b
     summed = {ix for ix in (a & b) if keep_counts[ix] == 2 and ix not in out_set}
     res = union - summed
-    return (1 << len(res)) - (1 << len(a)) - (1 << len(b)), frozenset(res)
+    return (1 << len(res)) - alpha * ((1 << len(a)) + (1 << len(b))), frozenset(res)
 
 
-def greedy_path(inputs, output):
+def greedy_path(inputs, output, alpha=1.0, temperature=0.0, seed=0):
     """SSA contraction path and the index set of every intermediate.
 
     ``inputs``: list of index lists; ``output``: list of kept indices.
     Returns (path, sets) where ``sets[id]`` is the frozenset of indices of
-    tensor ``id`` (inputs first, then one per step).
+    tensor ``id`` (inputs first, then one per step).  ``alpha`` weighs the
+    operand sizes in the cost (the rule above is alpha = 1); a positive
+    ``temperature`` perturbs each candidate's cost by seeded Gumbel noise
+    scaled by its magnitude, so that a set of seeds explores nearby orders.
     """
+    rng = random.Random(seed)
+
+    def noisy(c):
+        if temperature <= 0.0 or c == 0:
+            return c
+        return c - temperature * abs(c) * math.log(-math.log(rng.random() or 1e-300))
+
     out_set = set(output)
     sets = [frozenset(t) for t in inputs]
     counts = {}
@@ -52,8 +64,8 @@ def greedy_path(inputs, output):
     def push(i, j):
         if i > j:
             i, j = j, i
-        c, _ = _cost(sets[i], sets[j], counts, out_set)
-        heapq.heappush(heap, (c, i, j))
+        c, _ = _cost(sets[i], sets[j], counts, out_set, alpha)
+        heapq.heappush(heap, (noisy(c), i, j))
 
     seen = set()
     for ix, hs in holders.items():
@@ -68,7 +80,7 @@ def greedy_path(inputs, output):
         c, i, j = heapq.heappop(heap)
         if i not in live or j not in live:
             continue
-        _, res = _cost(sets[i], sets[j], counts, out_set)
+        _, res = _cost(sets[i], sets[j], counts, out_set, alpha)
         new = len(sets)
         sets.append(res)
         path.append((i, j))
@@ -97,6 +109,60 @@ def greedy_path(inputs, output):
         path.append((min(i, j), max(i, j)))
         rest = sorted(rest[2:] + [new], key=lambda t: (len(sets[t]), t))
     return path, sets
+
+
+def sweep_path(inputs, output):
+    """Contract the tensors into one accumulator in input order (circuit order)."""
+    out_set = set(output)
+    sets = [frozenset(t) for t in inputs]
+    counts = {}
+    for t in sets:
+        for ix in t:
+            counts[ix] = counts.get(ix, 0) + 1
+    path = []
+    acc = 0
+    for t in range(1, len(inputs)):
+        a, b = sets[acc], sets[t]
+        _, res = _cost(a, b, counts, out_set)
+        for ix in a | b:
+            counts[ix] -= (ix in a) + (ix in b)
+        for ix in res:
+            counts[ix] += 1
+        sets.append(res)
+        path.append((acc, t))
+        acc = len(sets) - 1
+    return path, sets
+
+
+# (alpha, temperature, seed) trials of the greedy rule, plus the sweep
+SEARCH_TRIALS = ((1.0, 0.0, 0), (2.0, 0.0, 0), (4.0, 0.0, 0), (8.0, 0.0, 0), (2.0, 0.05, 0), (2.0, 0.05, 1),
+                 (8.0, 0.05, 0), (1.0, 0.2, 0))
+
+
+def search_path(inputs, output, trials=SEARCH_TRIALS):
+    """Deterministic path search: the cheapest of a fixed set of candidate orders.
+
+    Candidates are the greedy rule under each (alpha, temperature, seed) of
+    ``trials`` and the circuit-order sweep; the winner minimises the number
+    of complex multiply-adds, ties broken by the largest intermediate.
+    Returns (path, sets, info).
+    """
+    n = len(inputs)
+    best = None
+    cands = [("sweep", None)] + [("greedy", t) for t in trials]
+    for kind, t in cands:
+        if n < 2:
+            path, sets = [], [frozenset(x) for x in inputs]
+        elif kind == "sweep":
+            path, sets = sweep_path(inputs, output)
+        else:
+            path, sets = greedy_path(inputs, output, *t)
+        largest, _ = path_stats(path, sets, n)
+        macs = sum(1 << len(sets[i] | sets[j]) for i, j in path)
+        key = (macs, largest)
+        if best is None or key < best[0]:
+            best = (key, path, sets, {"strategy": kind, "trial": t, "macs": macs, "largest_rank": largest})
+    return best[1], best[2], best[3]
 
 
 def path_stats(path, sets, n_inputs):

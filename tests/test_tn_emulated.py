@@ -1,0 +1,181 @@
+"""Contraction-mode host logic on the CPU: descriptors, light-cone reduction,
+paths and the reverse contraction tree, with the device calls emulated in
+numpy (tests/tn_emulator.py) and checked against the complex128 oracle.
+
+Tolerances: values 2e-5 and gradients 1e-4 absolute (complex64 leaves and
+single-precision storage of intermediates against complex128).
+"""
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import torch
+
+import tedq_amd as qai
+from oracle import sv_oracle as O
+from tedq_amd.tn.executor import TNExecutor, make_desc
+from tedq_amd.tn.network import cancel_inverse_pairs, gen_tensor_networks  # noqa: F401
+from tedq_amd.tn.path import greedy_path, search_path, sweep_path
+from tn_emulator import contract_np, install
+
+
+def test_descriptor_contraction_matches_einsum():
+    rng = np.random.default_rng(0)
+    letters = "abcdefgh"
+    for trial in range(20):
+        syms = list(letters)
+        rng.shuffle(syms)
+        na, nb = int(rng.integers(1, 6)), int(rng.integers(1, 6))
+        la = syms[:na]
+        lb = list(rng.choice(la, size=int(rng.integers(0, na + 1)), replace=False)) + syms[na:na + nb]
+        rng.shuffle(lb)
+        shared = [x for x in la if x in lb]
+        keep_shared = [x for x in shared if rng.random() < 0.3]
+        out = [x for x in la if x not in shared] + [x for x in lb if x not in shared] + keep_shared
+        rng.shuffle(out)
+        A = (rng.normal(size=2 ** len(la)) + 1j * rng.normal(size=2 ** len(la)))
+        B = (rng.normal(size=2 ** len(lb)) + 1j * rng.normal(size=2 ** len(lb)))
+        C = np.zeros((1, 2 ** len(out)), complex)
+        contract_np(make_desc(la, lb, out), A[None], B[None], C)
+        ref = np.einsum("".join(la) + "," + "".join(lb) + "->" + "".join(out),
+                        A.reshape([2] * len(la)), B.reshape([2] * len(lb)))
+        assert np.allclose(C.reshape(-1), np.asarray(ref).reshape(-1)), trial
+
+
+def test_paths_cover_every_tensor_once():
+    inputs = [["a", "b"], ["b", "c"], ["c", "d"], ["d", "a"], ["e"], ["e", "f"]]
+    for path, sets in (greedy_path(inputs, ["f"]), sweep_path(inputs, ["f"]), search_path(inputs, ["f"])[:2]):
+        used = [x for st in path for x in st]
+        assert sorted(used) == list(range(len(inputs) + len(path) - 1))
+        assert sets[-1] == frozenset({"f"})
+
+
+def _random_circuit(rng, n, depth):
+    """A random circuit as (frontend builder, oracle gate list as a function of the parameter vector)."""
+    spec = []
+    for _ in range(depth):
+        kind = rng.choice(["RX", "RY", "RZ", "Rot", "PhaseShift", "CRX", "CRZ", "CNOT", "CZ", "Hadamard", "S",
+                           "Toffoli", "SWAP", "U2", "RXinv"])
+        qs = list(rng.choice(n, size=3, replace=False))
+        spec.append((str(kind), qs))
+    u2 = sl.expm(-0.3j * np.kron(np.array([[0, 1], [1, 0]]), np.array([[1, 0], [0, -1]])))
+    npar = {"RX": 1, "RY": 1, "RZ": 1, "Rot": 3, "PhaseShift": 1, "CRX": 1, "CRZ": 1, "RXinv": 1}
+
+    def build(*p):
+        it = iter(p)
+        for kind, qs in spec:
+            if kind == "U2":
+                qai.Unitary(u2, qubits=qs[:2])
+            elif kind == "RXinv":
+                qai.RX(next(it), qubits=qs[:1]).inverse()
+            elif kind in ("Toffoli",):
+                qai.Toffoli(qubits=qs[:3])
+            elif kind in ("CRX", "CRZ", "CNOT", "CZ", "SWAP"):
+                getattr(qai, kind)(*[next(it) for _ in range(npar.get(kind, 0))], qubits=qs[:2])
+            else:
+                getattr(qai, kind)(*[next(it) for _ in range(npar.get(kind, 0))], qubits=qs[:1])
+
+    def gates(theta):
+        it = iter(theta)
+        out = []
+        for kind, qs in spec:
+            if kind == "U2":
+                out.append(("Unitary", qs[:2], u2))
+            elif kind == "RXinv":
+                out.append(("RX", qs[:1], [-next(it)]))
+            elif kind == "Toffoli":
+                out.append(("Toffoli", qs[:3], []))
+            elif kind in ("CRX", "CRZ", "CNOT", "CZ", "SWAP"):
+                out.append((kind, qs[:2], [next(it) for _ in range(npar.get(kind, 0))]))
+            else:
+                out.append((kind, qs[:1], [next(it) for _ in range(npar.get(kind, 0))]))
+        return out
+
+    P = sum(npar.get(k, 0) for k, _ in spec)
+    return build, gates, P
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_emulated_executor_values_and_gradients(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(3, 7))
+    build, gates, P = _random_circuit(rng, n, int(rng.integers(8, 20)))
+    q0, q1 = [int(x) for x in rng.choice(n, size=2, replace=False)]
+    herm = np.array([[1.0, 0.5 - 0.2j], [0.5 + 0.2j, -0.3]])
+    meas_kind = seed % 3
+
+    def cd(*p):
+        build(*p)
+        if meas_kind == 0:
+            return [qai.expval(qai.PauliZ(qubits=[q0])), qai.expval(qai.PauliX(qubits=[q1]))]
+        if meas_kind == 1:
+            return [qai.expval([qai.PauliZ(qubits=[q0]), qai.PauliY(qubits=[q1])]),
+                    qai.expval(qai.Unitary(herm, qubits=[q1]))]
+        return [qai.probs(qubits=sorted([q0, q1]))]
+
+    theta = rng.uniform(0, 2 * np.pi, size=max(P, 1))[:P]
+    cc = qai.Circuit(cd, n, *[torch.tensor(t) for t in theta]).compilecircuit(backend="pytorch", tn_mode=True)
+    ex = install(TNExecutor(cc, max_bytes=(1 << 40) if seed % 2 else 64))   # odd: full storage, even: segments
+    rows = torch.tensor(np.stack([theta, theta[::-1].copy()]), dtype=torch.float32)
+    ups = [torch.ones(2) if s.kind == "fexp" else torch.ones(2, 4) for s in ex.specs]
+    outs, grad = ex.value_and_grad(rows, ups)
+    for b in range(2):
+        th = torch.tensor(rows[b].numpy(), dtype=torch.float64, requires_grad=True)
+        if meas_kind == 0:
+            ms = [("expval", ("PauliZ", [q0])), ("expval", ("PauliX", [q1]))]
+        elif meas_kind == 1:
+            ms = [("expval", [("PauliZ", [q0]), ("PauliY", [q1])]), ("expval", (herm, [q1]))]
+        else:
+            ms = [("probs", sorted([q0, q1]))]
+        res = O.measure(O.propagate(n, gates(list(th))), n, ms)
+        for o, r in zip(outs, res):
+            got = o[b].numpy().reshape(-1)
+            want = r.detach().numpy().reshape(-1)
+            assert np.max(np.abs(got - want)) < 2e-5
+        if P:
+            tot = sum(r.sum() for r in res)
+            (g,) = torch.autograd.grad(tot, th)
+            assert np.max(np.abs(grad[b].numpy() - g.numpy())) < 1e-4
+
+
+def test_light_cone_reduction_keeps_value_and_shrinks_network():
+    n, steps = 10, 2
+
+    def cd(*p):
+        it = iter(p)
+        for q in range(1, n, 2):
+            qai.PauliX(qubits=[q])
+        for _ in range(steps):
+            for q in range(n):
+                qai.RZ(next(it), qubits=[q])
+            for q in range(n - 1):
+                qai.CNOT(qubits=[q, q + 1])
+                qai.RY(next(it), qubits=[q + 1])
+                qai.CNOT(qubits=[q, q + 1])
+        return qai.expval(qai.PauliZ(qubits=[4]))
+
+    P = steps * (2 * n - 1)
+    rng = np.random.default_rng(5)
+    theta = rng.uniform(0, 6.28, P)
+    cc = qai.Circuit(cd, n, *[torch.tensor(t) for t in theta]).compilecircuit(backend="pytorch", tn_mode=True)
+    tn = cc._tensor_networks[0]  # pylint: disable=protected-access
+    red = cancel_inverse_pairs(tn)
+    assert red.num_tensors < tn.num_tensors
+    ex = install(TNExecutor(cc))
+    outs, grad = ex.value_and_grad(torch.tensor(theta[None], dtype=torch.float32), [torch.ones(1)])
+    th = torch.tensor(theta, requires_grad=True)
+    gl = []
+    it = iter(th)
+    for q in range(1, n, 2):
+        gl.append(("PauliX", [q], []))
+    for _ in range(steps):
+        for q in range(n):
+            gl.append(("RZ", [q], [next(it)]))
+        for q in range(n - 1):
+            gl += [("CNOT", [q, q + 1], []), ("RY", [q + 1], [next(it)]), ("CNOT", [q, q + 1], [])]
+    ref = O.execute(n, gl, [("expval", ("PauliZ", [4]))])
+    ref.sum().backward()
+    assert abs(float(outs[0][0]) - float(ref)) < 2e-5
+    assert np.max(np.abs(grad[0].numpy() - th.grad.numpy())) < 1e-4
+    # gates after the last interaction with qubit 4's past cancel: their gradients are exactly zero
+    assert np.any(grad[0].numpy() == 0.0)
