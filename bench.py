@@ -218,6 +218,68 @@ def serving_leg(n_req, warm):
             "stream": "Poisson at 0.7x closed-loop capacity; n~U{6..24}, layers~U{2..20}; <Z_0> + full gradient"}
 
 
+def measured_traffic(kernel, n, layers, batch, ex):
+    """HBM bytes per launch of ``kernel`` from the committed FETCH_SIZE / WRITE_SIZE counter runs of this
+    workload (profiles/sv_pass_traffic_hea20_b256.json), when the run uses the profiled shapes; else None."""
+    path = os.path.join(ROOT, "profiles", "sv_pass_traffic_hea20_b256.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    w = rec["workload"]
+    same = (rec["kernel"] == kernel and w["qubits"] == n and w["layers"] == layers and w["batch"] == batch
+            and w["tile_bits_bwd"] == ex.bwd.plan.k and w["reg_bits_bwd"] == ex.bwd.plan.r)
+    return rec["traffic_bytes_per_launch"] if same else None
+
+
+def tn_leg(c4=True):
+    """Contraction mode: the 40-qubit disordered chain (forward, batch 32) and the 5x5 lattice (fwd + grad)."""
+    from tedq_amd import workloads as W  # pylint: disable=import-outside-toplevel
+    dev = torch.device("cuda", 0)
+    out = {}
+    t0 = time.perf_counter()
+    ex = W.disordered_chain(40, 8, 20, tn_mode=True).executor()
+    build_s = time.perf_counter() - t0
+    rng = np.random.default_rng(2)
+    rows = torch.tensor(W.chain_rows(rng.uniform(-5, 5, size=(32, 40)), 8), device=dev)
+    with torch.no_grad():
+        ex.measure(ex.run_forward(rows))
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            v = ex.measure(ex.run_forward(rows))[0]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+    info = ex.path_report()[0]
+    out["chain40"] = {"workload": "1-D disordered chain 40q, 8 Trotter steps, <Z_20>, batch 32 seeds, forward",
+                      "evals_per_s": 32 / dt, "ms_per_batch": dt * 1e3, "path_build_s": build_s,
+                      "steps": info["steps"], "largest_rank": info["largest_rank"], "macs_per_eval": info["macs"],
+                      "achieved_TFLOPs": 8.0 * info["macs"] * 32 / dt / 1e12,
+                      "mean_value": float(v.mean())}
+    if c4:
+        t0 = time.perf_counter()
+        ex = W.lattice(5, 6, tn_mode=True).executor()
+        build_s = time.perf_counter() - t0
+        rng = np.random.default_rng(3)
+        rows = torch.tensor(rng.uniform(0, 2 * np.pi, size=(1, 300)), dtype=torch.float32, device=dev)
+        ups = [torch.full((1,), 1.0 / 25, device=dev) for _ in range(25)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs, grad = ex.value_and_grad(rows, ups)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rep = ex.path_report()
+        out["lattice5x5"] = {"workload": "5x5 lattice 6 layers, mean <Z_i> over 25 networks, fwd + reverse-tree grad",
+                             "s_per_eval": dt, "path_build_s": build_s,
+                             "steps": sum(r["steps"] for r in rep),
+                             "largest_rank": max(r["largest_rank"] for r in rep),
+                             "macs_fwd": sum(r["macs"] for r in rep),
+                             "cost": float(sum(float(o[0]) for o in outs) / 25),
+                             "grad_finite": bool(torch.isfinite(grad).all())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,10 +290,14 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--tile-bits", type=int, default=12)
     ap.add_argument("--reg-bits", type=int, default=4)
+    ap.add_argument("--tile-bits-bwd", type=int, default=None)
+    ap.add_argument("--reg-bits-bwd", type=int, default=None)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--skip-tn", action="store_true")
+    ap.add_argument("--skip-c4", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for traffic)")
@@ -255,7 +321,8 @@ def main():
     rng = np.random.default_rng(1)
     theta = rng.uniform(0, 2 * np.pi, size=(B, L * n * 2))
     compiled = hea_circuit(n, L).compilecircuit(backend="pytorch", tile_bits=args.tile_bits,
-                                                 reg_bits=args.reg_bits)
+                                                 reg_bits=args.reg_bits, tile_bits_bwd=args.tile_bits_bwd,
+                                                 reg_bits_bwd=args.reg_bits_bwd)
     ex = compiled.executor()
     rows = torch.tensor(theta, dtype=torch.float32, device=dev)
 
@@ -309,12 +376,15 @@ def main():
                    "passes": ex.n_passes, "passes_bwd": len(ex.bwd.plan.passes)},
         "autotune": tuning,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, n, L, B, ex),
                      "bytes_per_launch": kd["bytes_per_launch"], "mean_launch_ms": kd["mean_ms"]},
         "kernels": kernels,
     }
     if not args.skip_serving:
         line["latency"] = serving_leg(args.serve_requests, warm=200)
+    if not args.skip_tn:
+        line["tensor_network"] = tn_leg(c4=not args.skip_c4)
     if not args.skip_plain:
         ps, cnt = time_plain_gpu(n, L, theta[:16], dev)
         line["plain_gpu"] = {"value": ps, "unit": "evals/s", "sample": f"{cnt} single-row calls",

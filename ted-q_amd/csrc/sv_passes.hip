@@ -397,7 +397,13 @@ __device__ __forceinline__ TileCtx make_ctx(const PassArgs& a, const int* __rest
     const int64_t tile = blockIdx.x;
     c.circuit = tile >> tpc_log;
     c.valid = true;
-    const uint64_t within = (uint64_t)(tile & ((1ll << tpc_log) - 1));
+    uint64_t within = (uint64_t)(tile & ((1ll << tpc_log) - 1));
+    // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
+    // When the tile's contiguous runs are shorter than a 128-byte line, the
+    // tiles differing in the lowest rest bit share lines; swapping that bit
+    // with rest bit 3 makes those partners blockIdx b and b^8, which land on
+    // the same XCD at about the same time, so the second read can hit L2.
+    if (tpc_log >= 4) within = (within & ~9ull) | ((within & 1ull) << 3) | ((within >> 3) & 1ull);
     c.gbase = (c.circuit << a.n) + (int64_t)pdep64(within, a.rest_mask);
     uint64_t lo = 0;
     for (int i = 0; i < a.tT; ++i)
