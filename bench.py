@@ -297,6 +297,7 @@ def main():
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--skip-tn", action="store_true")
+    ap.add_argument("--no-jit", action="store_true", help="keep the interpreted pass kernels")
     ap.add_argument("--skip-c4", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -332,6 +333,13 @@ def main():
         res = autotune(ex, rows)
         tuning = {"best": list(res["best"]), "fwd_ms": {f"{k}/{r}": v for (k, r), v in res["fwd"].items()},
                   "bwd_ms": {f"{k}/{r}": v for (k, r), v in res["bwd"].items()}}
+
+    jit_s = None
+    if not args.no_jit:
+        t0 = time.perf_counter()
+        jit_used = ex.enable_jit(dev)
+        jit_s = time.perf_counter() - t0
+        ex.jit = True
 
     def step():
         r = rows.detach().requires_grad_(True)
@@ -373,7 +381,9 @@ def main():
                                "fwd + adjoint grad", "qubits": n, "layers": L, "batch": B,
                    "params_per_circuit": 2 * n * L, "tile_bits": ex.fwd.plan.k, "reg_bits": ex.fwd.plan.r,
                    "tile_bits_bwd": ex.bwd.plan.k, "reg_bits_bwd": ex.bwd.plan.r,
-                   "passes": ex.n_passes, "passes_bwd": len(ex.bwd.plan.passes)},
+                   "passes": ex.n_passes, "passes_bwd": len(ex.bwd.plan.passes),
+                   "pass_kernels": "interpreted" if args.no_jit else
+                   {"generated_fwd": jit_used[0], "generated_bwd": jit_used[1], "compile_s": jit_s}},
         "autotune": tuning,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS,

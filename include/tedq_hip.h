@@ -111,6 +111,27 @@ int tedq_sv_axpy(void* lam, const void* G, int64_t batch, int n_qubits, float sc
 /* out[b][m] = compensated sum over chunks of partial[b][chunk][m]. */
 int tedq_sum_partials(const double* partial, int64_t batch, int64_t nchunk, int M, double* out, void* stream);
 
+/* ---- run-time specialised pass kernels ------------------------------------- */
+/* Compile generated pass kernels (one per pass, ops and offsets as constants;
+ * tedq_amd/engine/jit.py) with hiprtc for gfx950 and load them as a module on
+ * the current device.  header_srcs/header_names: the texts and include names
+ * of tedq_common.h and sv_ops.h.  Allocates host memory and device code
+ * memory (the module), no data buffers.  Same pass semantics as
+ * tedq_sv_apply_pass / tedq_sv_adjoint_pass, i.e. it replaces the same
+ * reference SV loop (pytorch_backend.py:358-380). */
+int tedq_jit_compile(const char* src, int n_headers, const char* const* header_srcs,
+                     const char* const* header_names, void** module_out);
+/* Kernel handle of `name` in a compiled module. */
+int tedq_jit_function(void* module, const char* name, void** fn_out);
+/* Unload a compiled module. */
+int tedq_jit_unload(void* module);
+/* Launch one generated pass (forward when adjoint == 0) over every circuit;
+ * arguments as tedq_sv_apply_pass / tedq_sv_adjoint_pass (k < n <= 28, at most
+ * 64 KiB of LDS per workgroup). */
+int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits,
+                  int reg_bits, uint64_t tile_mask, const void* mats, int64_t mat_stride, void* partials,
+                  int64_t partial_rows, int n_gslots, void* stream);
+
 /* ---- tensor-network executor ----------------------------------------------- */
 /* desc: 4 + 9*32 int32 — bit counts of the index roles (h kept+shared, m kept
  * only in A, n kept only in B, k summed), then for each of A_h A_m A_k B_h B_k

@@ -42,7 +42,9 @@ def build(verbose=True):
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = OUT + ".tmp"
-    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
+    # hiprtc compiles the per-circuit pass kernels at run time (csrc/sv_jit.hip)
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp,
+                    "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"], check=True)
     os.replace(tmp, OUT)
     for obj in objs:
         os.remove(obj)

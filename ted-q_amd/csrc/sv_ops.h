@@ -1,0 +1,185 @@
+// Device pieces shared by the pass kernels compiled into the library
+// (csrc/sv_passes.hip) and the per-circuit pass kernels generated at run time
+// (tedq_amd/engine/jit.py, compiled with hiprtc): op codes, pass arguments,
+// register-level gate templates, buffer access, tile placement and the wave
+// reduction of gradient bilinears.
+#pragma once
+
+#include "tedq_common.h"
+
+namespace tedq {
+
+enum OpCode : int {
+  kOpU1 = 1, kOpD1 = 2, kOpCU1 = 3, kOpCX = 4, kOpCZ = 5,  // 6: unused
+  kOpGrad1 = 7, kOpCGrad1 = 8, kOpCD1 = 9,
+};
+
+constexpr int kHdrTileBits = 4;
+constexpr int kHdrHiOff = 36;
+constexpr int kHdrStages = 100;
+constexpr int kStageWords = 48;
+
+struct PassArgs {
+  int64_t mat_stride;
+  int64_t batch;
+  int64_t total_tiles;      // batch * 2^(n-k)
+  uint64_t rest_mask;       // physical bits (< n) that are not tile bits
+  int n;
+  int k;
+  int tT;                   // log2(threads per tile)
+  int gT;                   // log2(tiles per workgroup)
+  int64_t np;               // adjoint: partials are [batch][n_gslots][np][8]               // partial rows per gradient slot per circuit
+  int n_gslots;
+};
+
+// ---------------------------------------------------------------------------
+// register-level gate kernels.  B is the target register bit and CM a control
+// mask over register bits, both compile-time, so every pair selection below
+// folds away: an op costs only its arithmetic (no per-pair scalar tests).
+// Ops with several control bits (rare: Toffoli) use the *_rt runtime forms.
+// ---------------------------------------------------------------------------
+template <int R, int B, int CM>
+struct U1Op {  // general 2x2 on bit B where all CM bits are 1
+  static __device__ __forceinline__ void run(float2 (&v)[R], float2 m0, float2 m1, float2 m2, float2 m3) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & (1 << B)) || (i & CM) != CM) continue;
+      const int j = i | (1 << B);
+      const float2 x = v[i], y = v[j];
+      v[i] = cmul2(m0, x, m1, y);
+      v[j] = cmul2(m2, x, m3, y);
+    }
+  }
+};
+
+template <int R, int B, int CM>
+struct DiagOp {  // diag(d0, d1) on bit B where all CM bits are 1
+  static __device__ __forceinline__ void run(float2 (&v)[R], float2 d0, float2 d1) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & CM) != CM) continue;
+      v[i] = cmul((i >> B) & 1 ? d1 : d0, v[i]);
+    }
+  }
+};
+
+template <int R, int B, int CM>
+struct CXOp {  // X on bit B where all CM bits are 1
+  static __device__ __forceinline__ void run(float2 (&v)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & (1 << B)) || (i & CM) != CM) continue;
+      const int j = i | (1 << B);
+      const float2 t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+    }
+  }
+};
+
+template <int R, int B, int CM>
+struct CZOp {  // -1 where bit B and all CM bits are 1
+  static __device__ __forceinline__ void run(float2 (&v)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if ((i & (1 << B)) && (i & CM) == CM) v[i] = cneg(v[i]);
+  }
+};
+
+// bilinears c_ab = sum conj(l_a) p_b over the pairs of bit B where all CM bits are 1
+template <int R, int B, int CM>
+struct GradOp {
+  static __device__ __forceinline__ void run(const float2 (&p)[R], const float2 (&l)[R], float (&acc)[8]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & (1 << B)) || (i & CM) != CM) continue;
+      const int j = i | (1 << B);
+      acc[0] = fmaf(l[i].x, p[i].x, fmaf(l[i].y, p[i].y, acc[0]));
+      acc[1] = fmaf(l[i].x, p[i].y, fmaf(-l[i].y, p[i].x, acc[1]));
+      acc[2] = fmaf(l[i].x, p[j].x, fmaf(l[i].y, p[j].y, acc[2]));
+      acc[3] = fmaf(l[i].x, p[j].y, fmaf(-l[i].y, p[j].x, acc[3]));
+      acc[4] = fmaf(l[j].x, p[i].x, fmaf(l[j].y, p[i].y, acc[4]));
+      acc[5] = fmaf(l[j].x, p[i].y, fmaf(-l[j].y, p[i].x, acc[5]));
+      acc[6] = fmaf(l[j].x, p[j].x, fmaf(l[j].y, p[j].y, acc[6]));
+      acc[7] = fmaf(l[j].x, p[j].y, fmaf(-l[j].y, p[j].x, acc[7]));
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// buffer access: the resource (tile base, uniform per workgroup) lives in
+// SGPRs, each lane adds a 32-bit offset and each register row a scalar offset
+// ---------------------------------------------------------------------------
+typedef int tedq_v2i __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  // raw buffer, stride 0, num_records in bytes; 0x00020000 is the gfx9 data-format word
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(float2 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tedq_v2i, v), r, voff, soff, 0);
+}
+
+// Circuit index and global element offset of tile `tile` of a pass whose
+// tiles are narrower than the circuit.  Workgroups are dealt round-robin to
+// the 8 XCDs (each with its own L2).  When the tile's contiguous runs are
+// shorter than a 128-byte line, the tiles differing in the lowest rest bit
+// share lines; swapping that bit with rest bit 3 makes those partners
+// blockIdx b and b^8, which land on the same XCD at about the same time, so
+// the second read can hit L2.
+__device__ __forceinline__ void tile_origin(const PassArgs& a, int64_t tile, int64_t* circuit, int64_t* gbase) {
+  const int64_t tpc_log = a.n - a.k;
+  *circuit = tile >> tpc_log;
+  uint64_t within = (uint64_t)(tile & ((1ll << tpc_log) - 1));
+  if (tpc_log >= 4) within = (within & ~9ull) | ((within & 1ull) << 3) | ((within >> 3) & 1ull);
+  *gbase = (*circuit << a.n) + (int64_t)pdep64(within, a.rest_mask);
+}
+
+// swizzled LDS index of this thread's register 0 in a stage
+__device__ __forceinline__ int stage_base(int tid, int tT, int k, int nonreg) {
+  const int g = tid >> tT;
+  int tl = tid & ((1 << tT) - 1);
+  int base = 0;
+  int mask = nonreg;
+  while (mask) {
+    const int low = mask & (-mask);
+    if (tl & 1) base |= low;
+    tl >>= 1;
+    mask &= mask - 1;
+  }
+  return swz((g << k) + base);
+}
+
+// Sum the 8 bilinears over the 64 lanes of a wave by recursive halving:
+// after the xor-32/16/8 steps lane L holds value (L >> 3) & 7 summed over lane
+// bits 3..5, then three butterfly steps finish bits 0..2 (10 shuffles instead
+// of 48).  Returns the total for value (lane >> 3) & 7, valid on lanes with
+// (lane & 7) == 0.
+__device__ __forceinline__ float reduce8_wave(float (&v)[8], int lane) {
+  float h4[4], h2[2];
+  const bool up5 = lane & 32;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float send = up5 ? v[j] : v[4 + j];
+    const float keep = up5 ? v[4 + j] : v[j];
+    h4[j] = keep + __shfl_xor(send, 32, 64);
+  }
+  const bool up4 = lane & 16;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const float send = up4 ? h4[j] : h4[2 + j];
+    const float keep = up4 ? h4[2 + j] : h4[j];
+    h2[j] = keep + __shfl_xor(send, 16, 64);
+  }
+  const bool up3 = lane & 8;
+  float x = (up3 ? h2[1] : h2[0]) + __shfl_xor(up3 ? h2[0] : h2[1], 8, 64);
+  x += __shfl_xor(x, 4, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 1, 64);
+  return x;
+}
+
+}  // namespace tedq

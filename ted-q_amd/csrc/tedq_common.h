@@ -1,8 +1,16 @@
 // Shared device helpers for the tedq MI355X engine (gfx950, wave64).
 #pragma once
 
+#if defined(__HIPCC_RTC__)  // hiprtc: the HIP device runtime is built in, the fixed-width types are not
+typedef signed char int8_t;
+typedef int int32_t;
+typedef unsigned int uint32_t;
+typedef long int64_t;
+typedef unsigned long uint64_t;
+#else
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace tedq {
 

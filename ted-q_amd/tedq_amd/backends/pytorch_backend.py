@@ -125,6 +125,8 @@ class PyTorchBackend(CompiledCircuit):
         self._tile_bits = kwargs.get("tile_bits", 12)
         self._reg_bits = kwargs.get("reg_bits", 4)
         self._tile_bits_bwd = kwargs.get("tile_bits_bwd", None)
+        # compile per-pass kernels for this circuit structure at first use (hiprtc, a few seconds)
+        self._jit = bool(kwargs.get("jit", False))
         self._reg_bits_bwd = kwargs.get("reg_bits_bwd", None)
         if self._interface != "pytorch":
             raise ValueError(f"{self._interface}: pytroch_backend only supports pytorch interface!")
@@ -202,6 +204,7 @@ class PyTorchBackend(CompiledCircuit):
                 self._executor = SVExecutor(n, units, n_params, pool, self._measurements, init_state=init,
                                             k_max=self._tile_bits, r=self._reg_bits,
                                             k_bwd=self._tile_bits_bwd, r_bwd=self._reg_bits_bwd)
+                self._executor.jit = self._jit
         return self._executor
 
     def _flatten_and_update(self, params):

@@ -1,0 +1,212 @@
+"""Per-circuit pass kernels: source generation and run-time compilation.
+
+The library's pass kernels interpret a pass program (op list, offsets,
+matrix positions) at run time.  For a circuit structure evaluated many
+times, this module turns the same encoded programs into one HIP kernel per
+pass in which every op, register bit, control mask, LDS offset and matrix
+position is a constant, and compiles them with hiprtc through the library
+(tedq_jit_compile, csrc/sv_jit.hip).  The generated code calls the device
+templates of csrc/sv_ops.h with the same arguments the interpreter derives
+from the program words, and reads and writes memory exactly as the
+interpreted kernels do (same tile placement, same LDS layout, same partial
+rows), so the two agree to rounding.
+
+Limits: tiles narrower than the circuit (k < n), n <= 28, and at most
+64 KiB of LDS per workgroup; other programs keep the interpreted kernels.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import hip
+from .planner import OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1
+from .program import HDR_TILE_BITS, HDR_HI_OFF, HDR_STAGES, STAGE_WORDS, OP_WORDS, lds_swizzle
+
+_CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))), "csrc")
+_HEADERS = ("tedq_common.h", "sv_ops.h")
+
+
+def supported(n, k, r, adjoint):
+    lds = (1 << k) * 8 * (2 if adjoint else 1)
+    return k < n <= 28 and lds <= 65536 and k - r <= 9 and 1 <= r <= 5
+
+
+def _bits(mask):
+    return [i for i in range(64) if (mask >> i) & 1]
+
+
+def _pdep_expr(var, positions):
+    """C expression depositing the low bits of ``var`` at ``positions`` (ascending bit i -> positions[i])."""
+    if not positions:
+        return "0u"
+    terms = []
+    for i, p in enumerate(positions):
+        if p >= i:
+            terms.append(f"((({var}) & {1 << i}u) << {p - i})")
+        else:
+            terms.append(f"((({var}) & {1 << i}u) >> {i - p})")
+    return " | ".join(terms)
+
+
+def _u1_args(dag, off):
+    m = [f"mb[{off + i}]" for i in range(4)]
+    if dag:
+        return f"cconj({m[0]}), cconj({m[2]}), cconj({m[1]}), cconj({m[3]})"
+    return ", ".join(m)
+
+
+def _diag_args(dag, off):
+    if dag:
+        return f"cconj(mb[{off}]), cconj(mb[{off + 3}])"
+    return f"mb[{off}], mb[{off + 3}]"
+
+
+def _op_lines(w, R, regs, adjoint, seg):
+    """Source lines of one op (8 program words) applied to the register arrays ``regs``."""
+    t, dag = int(w[0]) & 0xFF, (int(w[0]) >> 8) & 1
+    b, cm, off, gslot = int(w[1]), int(w[3]), int(w[4]), int(w[5])
+    out = []
+    if t in (OP_U1, OP_CU1):
+        out.append("{")
+        for v in regs:
+            out.append(f"  U1Op<{R}, {b}, {cm}>::run({v}, {_u1_args(dag, off)});")
+        out.append("}")
+    elif t in (OP_D1, OP_CD1):
+        for v in regs:
+            out.append(f"DiagOp<{R}, {b}, {cm}>::run({v}, {_diag_args(dag, off)});")
+    elif t == OP_CX:
+        for v in regs:
+            out.append(f"CXOp<{R}, {b}, {cm}>::run({v});")
+    elif t == OP_CZ:
+        mask = cm
+        top = mask.bit_length() - 1
+        for v in regs:
+            out.append(f"CZOp<{R}, {top}, {mask & ~(1 << top)}>::run({v});")
+    elif t in (OP_GRAD1, OP_CGRAD1):
+        if not adjoint:
+            raise ValueError("gradient op in a forward program")
+        out.append("{")
+        out.append("  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};")
+        out.append(f"  GradOp<{R}, {b}, {cm}>::run(vp, vl, acc);")
+        out.append(f"  float* dst = pbase + (int64_t){gslot} * a.np * 8;")
+        if seg == 64:
+            out.append("  const float tot = reduce8_wave(acc, lane);")
+            out.append("  if ((lane & 7) == 0) dst[(lane >> 3) & 7] = tot;")
+        else:
+            out.append("#pragma unroll")
+            out.append("  for (int q = 0; q < 8; ++q)")
+            out.append(f"    for (int o = 1; o < {seg}; o <<= 1) acc[q] += __shfl_xor(acc[q], o, 64);")
+            out.append(f"  if ((tid & {seg - 1}) == 0) {{")
+            out.append("    float4* d4 = reinterpret_cast<float4*>(dst);")
+            out.append("    d4[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);")
+            out.append("    d4[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);")
+            out.append("  }")
+        out.append("}")
+    else:
+        raise ValueError(f"op type {t} has no generated form")
+    return out
+
+
+def pass_kernel_source(name, words, adjoint):
+    """HIP source of one generated pass kernel from its program words (program.py layout)."""
+    words = np.asarray(words, dtype=np.int32)
+    k, r, nst, tT = (int(x) for x in words[:4])
+    R, T = 1 << r, 1 << tT
+    tile_bits = [int(x) for x in words[HDR_TILE_BITS:HDR_TILE_BITS + k]]
+    hi = words[HDR_HI_OFF:HDR_HI_OFF + 64].view(np.int64)
+    seg = min(T, 64)
+    W = max(T >> 6, 1)
+    L = [f'extern "C" __global__ void __launch_bounds__({T}) {name}(',
+         "    float2* __restrict__ psi, float2* __restrict__ lam, const float2* __restrict__ mats,",
+         "    float* __restrict__ partials, tedq::PassArgs a) {",
+         "  using namespace tedq;",
+         "  extern __shared__ float2 lds[];",
+         "  const int tid = threadIdx.x;",
+         "  int64_t circ, gbase;",
+         "  tile_origin(a, blockIdx.x, &circ, &gbase);",
+         "  const float2* __restrict__ mb = mats + circ * a.mat_stride;",
+         f"  const uint32_t lo = {_pdep_expr('(uint32_t)tid', tile_bits[:tT])};",
+         "  const int sw = swz(tid);",
+         "  const __amdgpu_buffer_rsrc_t rp = make_rsrc(psi + gbase, (uint32_t)(8u << a.n));"]
+    if adjoint:
+        L += ["  const __amdgpu_buffer_rsrc_t rl = make_rsrc(lam + gbase, (uint32_t)(8u << a.n));",
+              f"  float2* ldl = lds + {T * R};",
+              "  const int lane = tid & 63;",
+              f"  const int64_t prow = ((int64_t)blockIdx.x & ((1ll << (a.n - a.k)) - 1)) * {W} + (tid >> 6);",
+              "  float* const pbase = partials + (circ * a.n_gslots * a.np + prow) * 8;",
+              "  (void)lane;"]
+    for m in range(R):
+        L.append(f"  lds[sw ^ {lds_swizzle(m * T)}] = bload(rp, lo * 8u, {int(hi[m]) * 8}u);")
+        if adjoint:
+            L.append(f"  ldl[sw ^ {lds_swizzle(m * T)}] = bload(rl, lo * 8u, {int(hi[m]) * 8}u);")
+    L.append("  __syncthreads();")
+    regs = ("vp", "vl") if adjoint else ("v",)
+    for s in range(nst):
+        base = HDR_STAGES + STAGE_WORDS * s
+        nops, op_off, nonreg = int(words[base]), int(words[base + 1]), int(words[base + 2])
+        offs = [int(x) for x in words[base + 4:base + 4 + R]]
+        L.append("  {")
+        L.append(f"    const int sb = swz({_pdep_expr('tid', _bits(nonreg))});")
+        for v in regs:
+            L.append(f"    float2 {v}[{R}];")
+        for m in range(R):
+            for v, arr in zip(regs, ("lds", "ldl")):
+                L.append(f"    {v}[{m}] = {arr}[sb ^ {offs[m]}];")
+        for o in range(nops):
+            w = words[op_off + o * OP_WORDS: op_off + (o + 1) * OP_WORDS]
+            L.extend("    " + x for x in _op_lines(w, R, regs, adjoint, seg))
+        for m in range(R):
+            for v, arr in zip(regs, ("lds", "ldl")):
+                L.append(f"    {arr}[sb ^ {offs[m]}] = {v}[{m}];")
+        L.append("    __syncthreads();")
+        L.append("  }")
+    for m in range(R):
+        L.append(f"  bstore(lds[sw ^ {lds_swizzle(m * T)}], rp, lo * 8u, {int(hi[m]) * 8}u);")
+        if adjoint:
+            L.append(f"  bstore(ldl[sw ^ {lds_swizzle(m * T)}], rl, lo * 8u, {int(hi[m]) * 8}u);")
+    L.append("}")
+    return "\n".join(L)
+
+
+def program_source(enc, n_fwd, n_adj):
+    """Generated source of every pass of an encoded plan: kernels f0.. (forward) and a0.. (adjoint)."""
+    parts = ['#include "sv_ops.h"', ""]
+    offs = list(enc.fwd_offsets) + list(enc.adj_offsets) + [len(enc.prog)]
+    starts = sorted(set(offs))
+    ends = {o: starts[i + 1] for i, o in enumerate(starts[:-1])}
+    for i in range(n_fwd):
+        o = enc.fwd_offsets[i]
+        parts.append(pass_kernel_source(f"f{i}", enc.prog[o:ends[o]], False))
+    for i in range(n_adj):
+        o = enc.adj_offsets[i]
+        parts.append(pass_kernel_source(f"a{i}", enc.prog[o:ends[o]], True))
+    return "\n\n".join(parts) + "\n"
+
+
+class JitModule:
+    """A compiled module of generated pass kernels on the current device."""
+
+    def __init__(self, source, names):
+        texts = []
+        for h in _HEADERS:
+            with open(os.path.join(_CSRC, h), "rb") as f:
+                texts.append(f.read() + b"\0")
+        srcs = (ctypes.c_char_p * len(texts))(*texts)
+        hnames = (ctypes.c_char_p * len(_HEADERS))(*[h.encode() for h in _HEADERS])
+        mod = ctypes.c_void_p()
+        hip.call("tedq_jit_compile", source.encode(), len(_HEADERS), srcs, hnames, ctypes.byref(mod))
+        self.module = mod
+        self.functions = {}
+        for name in names:
+            fn = ctypes.c_void_p()
+            hip.call("tedq_jit_function", mod, name.encode(), ctypes.byref(fn))
+            self.functions[name] = fn
+
+    def __del__(self):
+        try:
+            if self.module:
+                hip.load().tedq_jit_unload(self.module)
+        except Exception:  # pylint: disable=broad-except
+            pass

@@ -114,6 +114,33 @@ class _Program:
         self.plan = build_plan(n, units, n_params, const_pool, k_max=k, r=r)
         self.enc = encode_plan(self.plan)
         self._dev = {}
+        self._jit = {}          # device key -> JitModule of generated pass kernels
+
+    def enable_jit(self, device, forward=True, adjoint=True):
+        """Compile per-pass kernels for this program on ``device`` (hiprtc); returns True if in use."""
+        from . import jit  # pylint: disable=import-outside-toplevel
+        key = (device.type, device.index)
+        k, r = self.plan.k, self.plan.r
+        fwd = forward and jit.supported(self.n, k, r, False)
+        adj = adjoint and jit.supported(self.n, k, r, True)
+        if not (fwd or adj):
+            return False
+        nf = len(self.enc.fwd_offsets) if fwd else 0
+        na = len(self.enc.adj_offsets) if adj else 0
+        src = jit.program_source(self.enc, nf, na)
+        with torch.cuda.device(device):
+            mod = jit.JitModule(src, [f"f{i}" for i in range(nf)] + [f"a{i}" for i in range(na)])
+        self._jit[key] = (mod, nf, na)
+        return True
+
+    def _jit_fns(self, device, adjoint):
+        got = self._jit.get((device.type, device.index))
+        if got is None:
+            return None
+        mod, nf, na = got
+        if adjoint:
+            return [mod.functions[f"a{i}"] for i in range(na)] if na else None
+        return [mod.functions[f"f{i}"] for i in range(nf)] if nf else None
 
     def buffers(self, device):
         key = (device.type, device.index)
@@ -156,11 +183,16 @@ class _Program:
         B = psi.shape[0]
         st = hip.stream_handle(dev)
         tl = self.threads_log2()
-        for off, mask in zip(self.enc.fwd_offsets, self.enc.tile_masks):
+        fns = self._jit_fns(dev, False)
+        for i, (off, mask) in enumerate(zip(self.enc.fwd_offsets, self.enc.tile_masks)):
             if probe:
                 probe.begin("sv_pass_fwd")
-            hip.call("tedq_sv_apply_pass", hip.ptr(psi), B, self.n, self.plan.k, self.plan.r, tl,
-                     buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, st)
+            if fns:
+                hip.call("tedq_jit_pass", fns[i], 0, hip.ptr(psi), None, B, self.n, self.plan.k, self.plan.r, mask,
+                         hip.ptr(mats), self.plan.mat_stride, None, 0, 0, st)
+            else:
+                hip.call("tedq_sv_apply_pass", hip.ptr(psi), B, self.n, self.plan.k, self.plan.r, tl,
+                         buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, st)
             if probe:
                 probe.end("sv_pass_fwd")
 
@@ -172,11 +204,17 @@ class _Program:
         tl = self.threads_log2()
         rows = self.partial_rows()
         G = self.plan.n_gslots
-        for off, mask in zip(self.enc.adj_offsets, reversed(self.enc.tile_masks)):
+        fns = self._jit_fns(dev, True)
+        for i, (off, mask) in enumerate(zip(self.enc.adj_offsets, reversed(self.enc.tile_masks))):
             if probe:
                 probe.begin("sv_pass_adj")
-            hip.call("tedq_sv_adjoint_pass", hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r, tl,
-                     buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, st)
+            if fns:
+                hip.call("tedq_jit_pass", fns[i], 1, hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
+                         mask, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, st)
+            else:
+                hip.call("tedq_sv_adjoint_pass", hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
+                         tl, buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows,
+                         G, st)
             if probe:
                 probe.end("sv_pass_adj")
 
@@ -204,10 +242,34 @@ class SVExecutor:
         self._dev = {}
         # optional timing probe with begin(kind)/end(kind) around every pass launch
         self.probe = None
+        # compile per-pass kernels on first use on a device (see enable_jit)
+        self.jit = False
+        self._jitted = set()
         self.configure(k_max, r, k_bwd, r_bwd)
+
+    def enable_jit(self, device=None):
+        """Compile the forward and adjoint programs into per-pass kernels (hiprtc) for ``device``.
+
+        Returns (forward_jitted, adjoint_jitted).  Reconfiguring the shapes drops the compiled kernels.
+        """
+        device = device or torch.device("cuda", torch.cuda.current_device())
+        if self.bwd is self.fwd:
+            both = self.fwd.enable_jit(device, True, True)
+            return both and self.fwd._jit_fns(device, False) is not None, \
+                both and self.fwd._jit_fns(device, True) is not None  # pylint: disable=protected-access
+        f = self.fwd.enable_jit(device, True, False)
+        b = self.bwd.enable_jit(device, False, True)
+        return f, b
+
+    def _ensure_jit(self, device):
+        key = (device.type, device.index)
+        if self.jit and key not in self._jitted:
+            self.enable_jit(device)
+            self._jitted.add(key)
 
     def configure(self, k, r, k_bwd=None, r_bwd=None):
         """(Re)build the forward and adjoint programs for the given tile / register shapes."""
+        self._jitted = set()
         k_bwd = k if k_bwd is None else k_bwd
         # the adjoint holds psi and lambda registers: more than 16 amplitudes each would spill
         r_bwd = min(4, r if r_bwd is None else r_bwd)
@@ -249,6 +311,7 @@ class SVExecutor:
         """Apply the circuit to every parameter row.  Returns psi [B, 2^n] complex64."""
         dev = params.device
         B = params.shape[0]
+        self._ensure_jit(dev)
         mats, _ = self.fwd.tables(params, False)
         psi = torch.empty((B, 2 ** self.n), dtype=torch.complex64, device=dev)
         st = hip.stream_handle(dev)
@@ -351,6 +414,7 @@ class SVExecutor:
         grad = torch.empty((B, max(self.n_params, 1)), dtype=grad_dtype, device=dev)
         if self.n_params == 0:
             return grad[:, :0]
+        self._ensure_jit(dev)
         prog = self.bwd
         mats, amats = prog.tables(params, True)
         G = prog.plan.n_gslots

@@ -279,3 +279,48 @@ def test_c2_full_size_properties():
     e0 = torch.zeros_like(work)
     e0[:, 0] = 1
     assert (work - e0).abs().max().item() < 1e-4
+
+
+@pytest.mark.parametrize("seed,n,tile,reg", [(0, 10, 6, 3), (1, 13, 8, 4), (2, 16, 12, 4), (3, 14, 9, 2)])
+def test_generated_pass_kernels_match_interpreter_and_oracle(seed, n, tile, reg):
+    """Per-pass kernels compiled at run time give the interpreted kernels' results (to rounding)
+    and the oracle's (1e-5 values, 1e-4 gradients)."""
+    print(f"case seed={seed} n={n} tile={tile} reg={reg}", flush=True)
+    rng = np.random.default_rng(50 + seed)
+    spec = _random_spec(n, 50, rng)
+    obs = [int(q) for q in rng.choice(n, size=3, replace=False)]
+    meas_fn = lambda: [qai.expval(qai.PauliZ(qubits=[q])) for q in obs]  # noqa: E731
+    c_int, npar = _build(n, spec, meas_fn, tile_bits=tile, reg_bits=reg)
+    c_jit, _ = _build(n, spec, meas_fn, tile_bits=tile, reg_bits=reg, jit=True)
+    vals = rng.uniform(-math.pi, math.pi, size=(5, npar))
+    rows = torch.tensor(vals, dtype=torch.float32, device=DEV)
+    ups = [torch.ones(5, device=DEV) for _ in obs]
+    ex_i, ex_j = c_int.executor(), c_jit.executor()
+    vi, gi = ex_i.value_and_grad(rows, ups)
+    vj, gj = ex_j.value_and_grad(rows, ups)
+    assert ex_j.fwd._jit_fns(DEV, False) is not None  # pylint: disable=protected-access
+    assert ex_j.bwd._jit_fns(DEV, True) is not None  # pylint: disable=protected-access
+    for a, b in zip(vi, vj):
+        assert np.max(np.abs(a.cpu().numpy() - b.cpu().numpy())) < 1e-6
+    assert np.max(np.abs(gi.cpu().numpy() - gj.cpu().numpy())) < 1e-5
+    t64 = torch.tensor(vals[0], dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, _oracle_gates(spec, list(t64)), [("expval", ("PauliZ", [q])) for q in obs])
+    ref.sum().backward()
+    got = np.array([float(v[0]) for v in vj])
+    assert np.max(np.abs(got - ref.detach().numpy())) < 1e-5
+    assert np.max(np.abs(gj[0].cpu().numpy() - t64.grad.numpy())) < 1e-4
+
+
+def test_generated_kernels_on_the_benchmark_circuit():
+    """HEA 20 qubits, 10 layers at the benchmark tile shapes: generated == interpreted kernels."""
+    from tedq_amd import workloads as W
+    kw = dict(tile_bits=12, reg_bits=4, tile_bits_bwd=11, reg_bits_bwd=3)
+    ex_i = W.hea(20, 10, **kw).executor()
+    ex_j = W.hea(20, 10, jit=True, **kw).executor()
+    rng = np.random.default_rng(1)
+    rows = torch.tensor(rng.uniform(0, 2 * np.pi, size=(8, 400)), dtype=torch.float32, device=DEV)
+    ups = [torch.ones(8, device=DEV) for _ in range(20)]
+    vi, gi = ex_i.value_and_grad(rows, ups)
+    vj, gj = ex_j.value_and_grad(rows, ups)
+    assert np.max(np.abs(torch.stack(vi).cpu().numpy() - torch.stack(vj).cpu().numpy())) < 1e-6
+    assert np.max(np.abs(gi.cpu().numpy() - gj.cpu().numpy())) < 1e-5
