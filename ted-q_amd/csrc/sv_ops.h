@@ -153,32 +153,51 @@ __device__ __forceinline__ int stage_base(int tid, int tT, int k, int nonreg) {
   return swz((g << k) + base);
 }
 
-// Sum the 8 bilinears over the 64 lanes of a wave by recursive halving:
-// after the xor-32/16/8 steps lane L holds value (L >> 3) & 7 summed over lane
-// bits 3..5, then three butterfly steps finish bits 0..2 (10 shuffles instead
-// of 48).  Returns the total for value (lane >> 3) & 7, valid on lanes with
-// (lane & 7) == 0.
+// Cross-lane moves without LDS: DPP within rows of 16 lanes, and the gfx950
+// permlane swaps across rows (16) and halves (32) of the wave.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
+}
+
+// The two results of a permlane swap must stay distinct values: this compiler
+// folds `s[0] + s[1]` of __builtin_amdgcn_permlane{16,32}_swap into s[0] + s[0]
+// (checked in the generated ISA).  An empty asm that takes both values as
+// read-write VGPR operands hides them from that fold; it emits no instruction.
+__device__ __forceinline__ void opaque2(unsigned& a, unsigned& b) { asm volatile("" : "+v"(a), "+v"(b)); }
+
+// Sum the 8 bilinears over the 64 lanes of a wave by recursive halving.  A
+// permlane32 swap exchanges the upper half of one value with the lower half
+// of another, so one add per pair leaves value j on lanes 0..31 and value j+4
+// on lanes 32..63; permlane16 does the same across 16-lane rows, and a DPP
+// rotate by 8 within rows finishes lane bit 3.  Then lane L holds value
+// (L >> 3) & 7 summed over lane bits 3..5, and DPP quad permutes plus a
+// half-row mirror sum lane bits 0..2.  All moves are VALU (no LDS round
+// trips).  Returns the total for value (lane >> 3) & 7, valid on every lane;
+// the wave must be fully active.
 __device__ __forceinline__ float reduce8_wave(float (&v)[8], int lane) {
   float h4[4], h2[2];
-  const bool up5 = lane & 32;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const float send = up5 ? v[j] : v[4 + j];
-    const float keep = up5 ? v[4 + j] : v[j];
-    h4[j] = keep + __shfl_xor(send, 32, 64);
+    const auto s = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, v[j]),
+                                                    __builtin_bit_cast(unsigned, v[4 + j]), false, false);
+    unsigned x0 = s[0], x1 = s[1];
+    opaque2(x0, x1);
+    h4[j] = __builtin_bit_cast(float, x0) + __builtin_bit_cast(float, x1);
   }
-  const bool up4 = lane & 16;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    const float send = up4 ? h4[j] : h4[2 + j];
-    const float keep = up4 ? h4[2 + j] : h4[j];
-    h2[j] = keep + __shfl_xor(send, 16, 64);
+    const auto s = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, h4[j]),
+                                                    __builtin_bit_cast(unsigned, h4[2 + j]), false, false);
+    unsigned x0 = s[0], x1 = s[1];
+    opaque2(x0, x1);
+    h2[j] = __builtin_bit_cast(float, x0) + __builtin_bit_cast(float, x1);
   }
   const bool up3 = lane & 8;
-  float x = (up3 ? h2[1] : h2[0]) + __shfl_xor(up3 ? h2[0] : h2[1], 8, 64);
-  x += __shfl_xor(x, 4, 64);
-  x += __shfl_xor(x, 2, 64);
-  x += __shfl_xor(x, 1, 64);
+  float x = (up3 ? h2[1] : h2[0]) + dpp_mov<0x128>(up3 ? h2[0] : h2[1]);  // row_ror:8
+  x += dpp_mov<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_mov<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_mov<0x141>(x);  // row_half_mirror: the other quad of the 8-lane group
   return x;
 }
 
