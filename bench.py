@@ -330,7 +330,7 @@ def main():
     tuning = None
     if not args.no_autotune:
         from tedq_amd.engine.autotune import autotune  # pylint: disable=import-outside-toplevel
-        res = autotune(ex, rows)
+        res = autotune(ex, rows, jit=not args.no_jit)
         tuning = {"best": list(res["best"]), "fwd_ms": {f"{k}/{r}": v for (k, r), v in res["fwd"].items()},
                   "bwd_ms": {f"{k}/{r}": v for (k, r), v in res["bwd"].items()}}
 

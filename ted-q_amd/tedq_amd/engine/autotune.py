@@ -32,28 +32,38 @@ def _time(fn, reps):
     return t0.elapsed_time(t1) / reps
 
 
-def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2):
+def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2, jit=False):
     """Configure ``ex`` (an SVExecutor) with the fastest shapes for ``rows`` ([B, P] on the GPU).
 
+    With ``jit`` every candidate runs its generated per-pass kernels (compiled
+    once per shape and kept by the executor); shapes those cannot serve are
+    timed with the interpreted kernels.
     Returns {"fwd": {shape: ms}, "bwd": {shape: ms}, "best": (k, r, k_bwd, r_bwd)}.
     """
     n = ex.n
+    dev = rows.device
     fwd_shapes = [s for s in fwd_shapes if s[0] <= n and valid_shape(*s, False)] or [(n, min(4, n))]
     bwd_shapes = [s for s in bwd_shapes if s[0] <= n and valid_shape(*s, True)] or [(n, min(4, n))]
-    k0, r0 = ex.fwd.plan.k, ex.fwd.plan.r
     kb0, rb0 = ex.bwd.plan.k, ex.bwd.plan.r
+    keep_jit = ex.jit
+    ex.jit = False              # compile explicitly below, only what each timing needs
     fwd_ms, bwd_ms = {}, {}
     for k, r in fwd_shapes:
         ex.configure(k, r, kb0, rb0)
+        if jit:
+            ex.fwd.enable_jit(dev, True, False)
         fwd_ms[(k, r)] = _time(lambda: ex.run_forward(rows), reps)
     kf, rf = min(fwd_ms, key=fwd_ms.get)
+    ex.configure(kf, rf, kb0, rb0)
     psi = ex.run_forward(rows)
     ups = [torch.ones(rows.shape[0], device=rows.device) for _ in ex.specs]
     lam = ex.seed_adjoint(psi, ups)
     for k, r in bwd_shapes:
         ex.configure(kf, rf, k, r)
+        if jit:
+            ex.bwd.enable_jit(dev, False, True)
         bwd_ms[(k, r)] = _time(lambda: ex.run_backward(rows, psi.clone(), lam.clone()), reps)
     kb, rb = min(bwd_ms, key=bwd_ms.get)
     ex.configure(kf, rf, kb, rb)
-    del k0, r0
+    ex.jit = keep_jit
     return {"fwd": fwd_ms, "bwd": bwd_ms, "best": (kf, rf, kb, rb)}

@@ -186,7 +186,11 @@ def program_source(enc, n_fwd, n_adj):
 
 
 class JitModule:
-    """A compiled module of generated pass kernels on the current device."""
+    """A compiled module of generated pass kernels on the current device.
+
+    Modules are never unloaded: launches are asynchronous, so a module dropped
+    by the host may still have kernels running.
+    """
 
     def __init__(self, source, names):
         texts = []
@@ -203,10 +207,3 @@ class JitModule:
             fn = ctypes.c_void_p()
             hip.call("tedq_jit_function", mod, name.encode(), ctypes.byref(fn))
             self.functions[name] = fn
-
-    def __del__(self):
-        try:
-            if self.module:
-                hip.load().tedq_jit_unload(self.module)
-        except Exception:  # pylint: disable=broad-except
-            pass

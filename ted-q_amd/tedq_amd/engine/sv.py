@@ -117,30 +117,32 @@ class _Program:
         self._jit = {}          # device key -> JitModule of generated pass kernels
 
     def enable_jit(self, device, forward=True, adjoint=True):
-        """Compile per-pass kernels for this program on ``device`` (hiprtc); returns True if in use."""
+        """Compile per-pass kernels for this program on ``device`` (hiprtc); returns True if any are in use.
+
+        Forward and adjoint kernels are compiled separately, once each; modules stay loaded for the
+        life of the program (kernels may still be running when a program is dropped).
+        """
         from . import jit  # pylint: disable=import-outside-toplevel
         key = (device.type, device.index)
         k, r = self.plan.k, self.plan.r
-        fwd = forward and jit.supported(self.n, k, r, False)
-        adj = adjoint and jit.supported(self.n, k, r, True)
-        if not (fwd or adj):
-            return False
-        nf = len(self.enc.fwd_offsets) if fwd else 0
-        na = len(self.enc.adj_offsets) if adj else 0
-        src = jit.program_source(self.enc, nf, na)
-        with torch.cuda.device(device):
-            mod = jit.JitModule(src, [f"f{i}" for i in range(nf)] + [f"a{i}" for i in range(na)])
-        self._jit[key] = (mod, nf, na)
-        return True
+        have = self._jit.setdefault(key, {"f": None, "a": None, "mods": []})
+        for kind, want, adj in (("f", forward, False), ("a", adjoint, True)):
+            if not want or have[kind] is not None or not jit.supported(self.n, k, r, adj):
+                continue
+            offs = self.enc.adj_offsets if adj else self.enc.fwd_offsets
+            names = [f"{kind}{i}" for i in range(len(offs))]
+            src = jit.program_source(self.enc, 0 if adj else len(offs), len(offs) if adj else 0)
+            with torch.cuda.device(device):
+                mod = jit.JitModule(src, names)
+            have["mods"].append(mod)
+            have[kind] = [mod.functions[nm] for nm in names]
+        return have["f"] is not None or have["a"] is not None
 
     def _jit_fns(self, device, adjoint):
         got = self._jit.get((device.type, device.index))
         if got is None:
             return None
-        mod, nf, na = got
-        if adjoint:
-            return [mod.functions[f"a{i}"] for i in range(na)] if na else None
-        return [mod.functions[f"f{i}"] for i in range(nf)] if nf else None
+        return got["a"] if adjoint else got["f"]
 
     def buffers(self, device):
         key = (device.type, device.index)
@@ -250,16 +252,13 @@ class SVExecutor:
     def enable_jit(self, device=None):
         """Compile the forward and adjoint programs into per-pass kernels (hiprtc) for ``device``.
 
-        Returns (forward_jitted, adjoint_jitted).  Reconfiguring the shapes drops the compiled kernels.
+        Returns (forward_jitted, adjoint_jitted).  Compiled programs are kept per (k, r) shape.
         """
         device = device or torch.device("cuda", torch.cuda.current_device())
-        if self.bwd is self.fwd:
-            both = self.fwd.enable_jit(device, True, True)
-            return both and self.fwd._jit_fns(device, False) is not None, \
-                both and self.fwd._jit_fns(device, True) is not None  # pylint: disable=protected-access
-        f = self.fwd.enable_jit(device, True, False)
-        b = self.bwd.enable_jit(device, False, True)
-        return f, b
+        self.fwd.enable_jit(device, True, False)
+        self.bwd.enable_jit(device, False, True)
+        # pylint: disable=protected-access
+        return self.fwd._jit_fns(device, False) is not None, self.bwd._jit_fns(device, True) is not None
 
     def _ensure_jit(self, device):
         key = (device.type, device.index)
@@ -267,17 +266,22 @@ class SVExecutor:
             self.enable_jit(device)
             self._jitted.add(key)
 
+    def _program(self, k, r):
+        """Fusion plan for (k, r), built once per executor (with its compiled kernels, if any)."""
+        if not hasattr(self, "_programs"):
+            self._programs = {}
+        if (k, r) not in self._programs:
+            self._programs[(k, r)] = _Program(self.n, self._units, self.n_params, list(self._pool), k, r)
+        return self._programs[(k, r)]
+
     def configure(self, k, r, k_bwd=None, r_bwd=None):
-        """(Re)build the forward and adjoint programs for the given tile / register shapes."""
+        """Select the forward and adjoint programs for the given tile / register shapes."""
         self._jitted = set()
         k_bwd = k if k_bwd is None else k_bwd
         # the adjoint holds psi and lambda registers: more than 16 amplitudes each would spill
         r_bwd = min(4, r if r_bwd is None else r_bwd)
-        self.fwd = _Program(self.n, self._units, self.n_params, list(self._pool), k, r)
-        if (k_bwd, r_bwd) == (k, r):
-            self.bwd = self.fwd
-        else:
-            self.bwd = _Program(self.n, self._units, self.n_params, list(self._pool), k_bwd, r_bwd)
+        self.fwd = self._program(k, r)
+        self.bwd = self._program(k_bwd, r_bwd)
 
     # ------------------------------------------------------------------
     @property
