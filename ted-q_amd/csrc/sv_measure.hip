@@ -83,6 +83,73 @@ __global__ void __launch_bounds__(kRedThreads) k_expval_z_partial(const float2* 
   }
 }
 
+// Same partial sums for n >= 14, organised so that the parity work is not per
+// amplitude: thread t of chunk c handles the 64 amplitudes x = c*2^14 + s*2^12
+// + j*2^8 + t (s < 4, j < 16).  For each s the 16 probabilities p_j are
+// Walsh-Hadamard transformed in registers, W[f] = sum_j p_j (-1)^popc(j & f),
+// and then mask m contributes (-1)^popc(hi & mask) * W[(mask >> 8) & 15] with hi
+// the bits of x outside j.  Per amplitude that is about 4 fp64 adds instead of
+// a popcount, select and add for every mask.  Each thread keeps its W in its
+// own LDS column, so the per-mask lookup is an indexed LDS read.
+constexpr int kWhtChunkLog = 14;
+static_assert(kWhtChunkLog == kChunkLog, "the transformed kernels use the reduction chunks");
+
+__global__ void __launch_bounds__(kRedThreads) k_expval_z_wht(const float2* __restrict__ st, int n,
+                                                               const uint64_t* __restrict__ masks, int M,
+                                                               double* __restrict__ partial, int64_t nchunk) {
+  __shared__ double wl[16][kRedThreads];
+  __shared__ double red[kRedThreads / 64][kMaxZ];
+  const int64_t b = blockIdx.y;
+  const int64_t chunk = blockIdx.x;
+  const int t = threadIdx.x;
+  const int64_t base = (b << n) + (chunk << kWhtChunkLog);
+  double acc[kMaxZ];
+#pragma unroll
+  for (int m = 0; m < kMaxZ; ++m) acc[m] = 0.0;
+  for (int s = 0; s < 4; ++s) {
+    double w[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float2 a = st[base + (s << 12) + (j << 8) + t];
+      w[j] = (double)a.x * a.x + (double)a.y * a.y;
+    }
+#pragma unroll
+    for (int h = 1; h < 16; h <<= 1)
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (!(j & h)) {
+          const double u = w[j], v = w[j | h];
+          w[j] = u + v;
+          w[j | h] = u - v;
+        }
+#pragma unroll
+    for (int f = 0; f < 16; ++f) wl[f][t] = w[f];
+    const uint64_t hi = ((uint64_t)chunk << kWhtChunkLog) | ((uint64_t)s << 12) | (uint64_t)t;
+#pragma unroll
+    for (int m = 0; m < kMaxZ; ++m) {
+      if (m < M) {
+        const uint64_t mk = masks[m];
+        const double v = wl[(mk >> 8) & 15][t];
+        acc[m] += (__popcll(hi & mk) & 1) ? -v : v;
+      }
+    }
+  }
+  const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+  for (int m = 0; m < kMaxZ; ++m) {
+    if (m < M) {
+      const double v = wave_sum_d(acc[m]);
+      if (lane == 0) red[wave][m] = v;
+    }
+  }
+  __syncthreads();
+  if (t < M) {
+    double sum = 0.0;
+    for (int w2 = 0; w2 < kRedThreads / 64; ++w2) sum += red[w2][t];
+    partial[(b * nchunk + chunk) * M + t] = sum;
+  }
+}
+
 // out[b][m] = compensated sum over chunks; one thread per (b, m)
 template <typename OT>
 __global__ void k_sum_partials(const double* __restrict__ partial, int64_t batch, int64_t nchunk, int M,
@@ -110,6 +177,49 @@ __global__ void k_lambda_z(const float2* __restrict__ psi, float2* __restrict__ 
   }
   const float2 a = psi[i];
   lam[i] = make_float2((float)(w * a.x), (float)(w * a.y));
+}
+
+// lambda for n >= 14 with the same layout as k_expval_z_wht: per thread and s,
+// E[f] = sum over masks with (mask >> 8) & 15 == f of dy_m (-1)^popc(hi & mask)
+// (built in the thread's LDS column), then a Walsh-Hadamard transform gives
+// w(x) for the 16 amplitudes x = hi | j*2^8 at once.
+__global__ void __launch_bounds__(kRedThreads) k_lambda_z_wht(const float2* __restrict__ psi,
+                                                               float2* __restrict__ lam, int n,
+                                                               const uint64_t* __restrict__ masks, int M,
+                                                               const double* __restrict__ dy) {
+  __shared__ double el[16][kRedThreads];
+  const int64_t b = blockIdx.y;
+  const int64_t chunk = blockIdx.x;
+  const int t = threadIdx.x;
+  const int64_t base = (b << n) + (chunk << kWhtChunkLog);
+  for (int s = 0; s < 4; ++s) {
+    const uint64_t hi = ((uint64_t)chunk << kWhtChunkLog) | ((uint64_t)s << 12) | (uint64_t)t;
+#pragma unroll
+    for (int f = 0; f < 16; ++f) el[f][t] = 0.0;
+    for (int m = 0; m < M; ++m) {
+      const uint64_t mk = masks[m];
+      const double d = dy[b * M + m];
+      el[(mk >> 8) & 15][t] += (__popcll(hi & mk) & 1) ? -d : d;
+    }
+    double e[16];
+#pragma unroll
+    for (int f = 0; f < 16; ++f) e[f] = el[f][t];
+#pragma unroll
+    for (int h = 1; h < 16; h <<= 1)
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (!(j & h)) {
+          const double u = e[j], v = e[j | h];
+          e[j] = u + v;
+          e[j | h] = u - v;
+        }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int64_t x = base + (s << 12) + (j << 8) + t;
+      const float2 a = psi[x];
+      lam[x] = make_float2((float)(e[j] * a.x), (float)(e[j] * a.y));
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -392,8 +502,12 @@ extern "C" int tedq_sv_expval_z(const void* state, int64_t batch, int n_qubits, 
   }
   const int64_t nchunk = tedq_sv_reduce_chunks(n_qubits);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_expval_z_partial, dim3((unsigned)nchunk, (unsigned)batch), dim3(kRedThreads), 0, st,
-                     static_cast<const float2*>(state), n_qubits, dev_masks, n_masks, partial, nchunk);
+  if (n_qubits >= kWhtChunkLog)
+    hipLaunchKernelGGL(k_expval_z_wht, dim3((unsigned)nchunk, (unsigned)batch), dim3(kRedThreads), 0, st,
+                       static_cast<const float2*>(state), n_qubits, dev_masks, n_masks, partial, nchunk);
+  else
+    hipLaunchKernelGGL(k_expval_z_partial, dim3((unsigned)nchunk, (unsigned)batch), dim3(kRedThreads), 0, st,
+                       static_cast<const float2*>(state), n_qubits, dev_masks, n_masks, partial, nchunk);
   int rc = check_launch("tedq_sv_expval_z(partial)");
   if (rc) return rc;
   hipLaunchKernelGGL(k_sum_partials<double>, dim3(nblocks(batch * n_masks, 128)), dim3(128), 0, st, partial, batch, nchunk,
@@ -409,9 +523,14 @@ extern "C" int tedq_sv_lambda_z(const void* psi, void* lam, int64_t batch, int n
     return kBadArgument;
   }
   const int64_t total = batch << n_qubits;
-  hipLaunchKernelGGL(k_lambda_z, dim3(nblocks(total, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const float2*>(psi), static_cast<float2*>(lam), total, n_qubits, dev_masks, n_masks,
-                     dy);
+  if (n_qubits >= kWhtChunkLog && batch <= 65535)
+    hipLaunchKernelGGL(k_lambda_z_wht, dim3((unsigned)tedq_sv_reduce_chunks(n_qubits), (unsigned)batch),
+                       dim3(kRedThreads), 0, static_cast<hipStream_t>(stream), static_cast<const float2*>(psi),
+                       static_cast<float2*>(lam), n_qubits, dev_masks, n_masks, dy);
+  else
+    hipLaunchKernelGGL(k_lambda_z, dim3(nblocks(total, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const float2*>(psi), static_cast<float2*>(lam), total, n_qubits, dev_masks,
+                       n_masks, dy);
   return check_launch("tedq_sv_lambda_z");
 }
 
