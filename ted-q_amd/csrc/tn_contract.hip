@@ -22,7 +22,8 @@
 //     additionally split over S workgroups whose partial sums are added in a
 //     fixed order by k_tn_split_sum (deterministic, no atomics).
 //   * k_tn_gemm: 64x64 output tiles, k in chunks of 16 staged through LDS,
-//     4x4 complex outputs per thread; used when m, n and k are all wide.
+//     complex products on the FP32 matrix cores (four real 16x16x4 MFMAs per
+//     complex block and k-step); used when m, n and k are all wide.
 // Arithmetic is complex64 with fp32 accumulation, like the reference's
 // complex64 torch contraction.
 #include "tedq_common.h"
@@ -152,9 +153,10 @@ __global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restri
   C[b * c_bs + (scat(d, CH, nH, h) | scat(d, CM, nM, m) | scat(d, CN, nN, n))] = acc;
 }
 
-// 64x64 output tile of one (batch, h); k staged 16 at a time through LDS.
+// 64x64 output tile of one (batch, h); k staged 16 at a time through LDS; MFMA.
 constexpr int kTile = 64;
 constexpr int kChunk = 16;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, const float2* __restrict__ B,
                                                  float2* __restrict__ C, const int32_t* __restrict__ dd,
@@ -183,12 +185,22 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
   }
   const float2* a = A + b * a_bs;
   const float2* bp = B + b * b_bs;
-  const int ty = t >> 4, tx = t & 15;
-  float2 acc[4][4];
+  // MFMA layout (v_mfma_f32_16x16x4_f32): lane l supplies A[l % 16][l / 16]
+  // and B[l / 16][l % 16] and receives D[4 (l / 16) + v][l % 16] in register v.
+  // Wave w owns output rows 32 (w & 1) .. +32 and columns 32 (w >> 1) .. +32
+  // as 2 x 2 blocks of 16 x 16; complex products as four real MFMAs
+  // (re += ar br - ai bi, im += ar bi + ai br).
+  const int lane = t & 63, w = t >> 6;
+  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  f32x4 cr[2][2], ci[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = make_float2(0.f, 0.f);
+    for (int j = 0; j < 2; ++j) {
+      cr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      ci[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
   const uint32_t ksz = 1u << nK;
   for (uint32_t k0 = 0; k0 < ksz; k0 += kChunk) {
     if (t >= 2 * kTile && t < 2 * kTile + kChunk) {
@@ -199,10 +211,13 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
       kb[t - 2 * kTile - kChunk] = scat(d, BK, nK, k < ksz ? k : 0);
     }
     __syncthreads();
+    // k runs fastest across threads: the host lays GEMM operands out with the
+    // summed indices as their lowest address bits, so 16 consecutive threads
+    // read one contiguous 128-byte run of a row
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < kTile * kChunk / 256; ++r) {
       const int e = t + 256 * r;
-      const int mi = e & (kTile - 1), kk = e >> 6;
+      const int kk = e & (kChunk - 1), mi = e >> 4;
       const bool ok = (m0 + mi < msz) && (k0 + kk < ksz);
       As[kk][mi] = ok ? a[rowA[mi] | ka[kk]] : make_float2(0.f, 0.f);
       const bool okb = (n0 + mi < nsz) && (k0 + kk < ksz);
@@ -210,33 +225,44 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < kChunk; ++kk) {
-      float2 x[4], y[4];
+    for (int ks = 0; ks < kChunk; ks += 4) {
+      float2 x[2], y[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) x[i] = As[kk][ty * 4 + i];
+      for (int i = 0; i < 2; ++i) x[i] = As[ks + lk][wm + 16 * i + lr];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) y[j] = Bs[kk][tx * 4 + j];
+      for (int j = 0; j < 2; ++j) y[j] = Bs[ks + lk][wn + 16 * j + lr];
+      // one term at a time over the four blocks, so that consecutive MFMAs
+      // never wait on each other's accumulator
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          acc[i][j].x = fmaf(x[i].x, y[j].x, fmaf(-x[i].y, y[j].y, acc[i][j].x));
-          acc[i][j].y = fmaf(x[i].x, y[j].y, fmaf(x[i].y, y[j].x, acc[i][j].y));
-        }
+        for (int j = 0; j < 2; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].x, cr[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].y, ci[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-x[i].y, y[j].y, cr[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].y, y[j].x, ci[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
   float2* c = C + b * c_bs;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const uint32_t mi = ty * 4 + i;
-    if (m0 + mi >= msz) continue;
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t ni = tx * 4 + j;
-      if (n0 + ni < nsz) c[rowC[mi] | colC[ni]] = acc[i][j];
-    }
-  }
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t mi = wm + 16 * i + 4 * lk + v;
+        const uint32_t ni = wn + 16 * j + lr;
+        if (m0 + mi < msz && n0 + ni < nsz) c[rowC[mi] | colC[ni]] = make_float2(cr[i][j][v], ci[i][j][v]);
+      }
 }
 
 // ---------------------------------------------------------------------------

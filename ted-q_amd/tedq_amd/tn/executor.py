@@ -52,6 +52,30 @@ def _bitpos(lst):
     return {s: r - 1 - i for i, s in enumerate(lst)}
 
 
+def gemm_shaped(nm, nn, nk):
+    """Steps the library runs on the matrix-core GEMM kernel (tn_contract.hip, plan_launch)."""
+    return nm >= 6 and nn >= 6 and nk >= 4
+
+
+def step_roles(la, lb, lc):
+    """Index roles of C = contract(A, B): kept-shared h, A-only m, B-only n, summed k (kernel bit order)."""
+    sa, sb, sc = set(la), set(lb), set(lc)
+    pa, pc = _bitpos(la), _bitpos(lc)
+    return {"h": sorted(sa & sb & sc, key=pc.get), "m": sorted((sa - sb) & sc, key=pc.get),
+            "n": sorted((sb - sa) & sc, key=pc.get), "k": sorted((sa & sb) - sc, key=pa.get)}
+
+
+def gemm_operand_layouts(la, lb, lc):
+    """For a GEMM-shaped step, operand index lists with the summed indices as the lowest address bits
+    (A as [h, m, k], B as [h, n, k], k in the same order in both) so that the kernel's tile loads read
+    contiguous runs along k; None for other steps."""
+    r = step_roles(la, lb, lc)
+    if not gemm_shaped(len(r["m"]), len(r["n"]), len(r["k"])):
+        return None
+    rev = lambda xs: list(reversed(xs))  # noqa: E731  (role bit 0 is the least significant address bit)
+    return rev(r["h"]) + rev(r["m"]) + rev(r["k"]), rev(r["h"]) + rev(r["n"]) + rev(r["k"])
+
+
 def make_desc(la, lb, lc):
     """Descriptor of C = contract(A, B) for index lists (most significant first) of A, B and C."""
     sa, sb, sc = set(la), set(lb), set(lc)
@@ -129,9 +153,9 @@ class _Network:
                 out = [x for x in la if x in res] + [x for x in lb if x in res and x not in la]
             lay.append(out)
             needs.append(needs[i] or needs[j])
-            fwd = desc_rows(make_desc(lay[i], lay[j], out))
-            ba = desc_rows(make_desc(out, lay[j], lay[i])) if needs[i] else -1
-            bb = desc_rows(make_desc(lay[i], out, lay[j])) if needs[j] else -1
+            fwd = desc_rows(lay[i], lay[j], out)
+            ba = desc_rows(out, lay[j], lay[i]) if needs[i] else -1
+            bb = desc_rows(lay[i], out, lay[j]) if needs[j] else -1
             self.steps.append(_Step(i, j, self.n_inputs + s, fwd, ba, bb))
         self.lay = lay
         self.ranks = [len(x) for x in lay]
@@ -213,8 +237,11 @@ class TNExecutor:
         self._const_len = 3
         descs = []
 
-        def desc_rows(d):
-            descs.append(d)
+        lists = []
+
+        def desc_rows(la, lb, lc):
+            descs.append(make_desc(la, lb, lc))
+            lists.append((list(la), list(lb), list(lc)))
             return len(descs) - 1
 
         self.networks = []
@@ -251,6 +278,20 @@ class TNExecutor:
                 self.specs.append(MeasSpec("state", tuple([2] * self.n)))
             else:
                 raise NotImplementedError(f"measurement {meas.return_type} in contraction mode")
+        # GEMM-shaped steps: copy operands whose summed indices are not already the lowest address bits
+        # into that layout first (a contraction with the scalar 1), then contract the copies
+        self._gemm = {}
+        for row, (la, lb, lc) in enumerate(list(lists)):
+            lay = gemm_operand_layouts(la, lb, lc)
+            if lay is None:
+                continue
+            ca, cb = lay
+            pa = desc_rows(la, [], ca) if ca != la else -1
+            pb = desc_rows(lb, [], cb) if cb != lb else -1
+            if pa < 0 and pb < 0:
+                continue
+            main = desc_rows(ca if pa >= 0 else la, cb if pb >= 0 else lb, lc)
+            self._gemm[row] = (pa, len(ca), pb, len(cb), main)
         self._consts = np.concatenate(consts).astype(np.complex64)
         self._descs = np.stack(descs).astype(np.int32) if descs else np.zeros((1, _DESC_WORDS), np.int32)
         self._dev = {}
@@ -291,6 +332,22 @@ class TNExecutor:
         return t, t.shape[1], 0
 
     def _contract(self, row, a, b, c, B, dev, buf):
+        g = self._gemm.get(row)
+        if g is not None:
+            pa, ra, pb, rb, main = g
+            one = (buf["consts"], 0, 2)
+            if pa >= 0:
+                t = torch.empty((B, 1 << ra), dtype=torch.complex64, device=dev)
+                self._contract_raw(pa, a, one, (t, t.shape[1], 0), B, dev, buf)
+                a = (t, t.shape[1], 0)
+            if pb >= 0:
+                t = torch.empty((B, 1 << rb), dtype=torch.complex64, device=dev)
+                self._contract_raw(pb, b, one, (t, t.shape[1], 0), B, dev, buf)
+                b = (t, t.shape[1], 0)
+            return self._contract_raw(main, a, b, c, B, dev, buf)
+        return self._contract_raw(row, a, b, c, B, dev, buf)
+
+    def _contract_raw(self, row, a, b, c, B, dev, buf):
         (ta, sa, oa), (tb, sb, ob), (tc, sc, oc) = a, b, c
         lib = hip.load()
         desc = self._descs[row]

@@ -128,7 +128,7 @@ def install(ex):
                 g[b, p] += float(np.real((Ek * dV).sum() + (Eb * dV.conj().T).sum()))
 
     ex._leaves = leaves  # pylint: disable=protected-access
-    ex._contract = contract  # pylint: disable=protected-access
+    ex._contract_raw = contract  # pylint: disable=protected-access
     ex._buffers = buffers  # pylint: disable=protected-access
     ex._param_grad = param_grad  # pylint: disable=protected-access
     return ex
