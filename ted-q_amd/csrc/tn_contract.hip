@@ -16,11 +16,14 @@
 // position in A, B or C), so any operand and result layout is allowed.
 //
 // Two kernels:
-//   * k_tn_small: a group of LPO lanes per output element strides over k
-//     (lanes adjacent in k, so an inner product over A's low bits reads
-//     contiguous memory), reduced with wave shuffles; for large k the range is
-//     additionally split over S workgroups whose partial sums are added in a
-//     fixed order by k_tn_split_sum (deterministic, no atomics).
+//   * k_tn_small: output elements in C's address order (coalesced writes), a
+//     group of LPO lanes per element strides over k (lanes adjacent in k, so
+//     an inner product over A's low bits reads contiguous memory), reduced with
+//     wave shuffles; for large k the range is additionally split over S
+//     workgroups whose partial sums are added in a fixed order by
+//     k_tn_split_sum (deterministic, no atomics).
+//   * k_tn_stream: the skinny steps (a large tensor times a gate), 16
+//     outputs per thread with table-driven operand offsets;
 //   * k_tn_gemm: 64x64 output tiles, k in chunks of 16 staged through LDS,
 //     complex products on the FP32 matrix cores (four real 16x16x4 MFMAs per
 //     complex block and k-step); used when m, n and k are all wide.
@@ -63,6 +66,37 @@ __device__ __forceinline__ uint32_t scat(const TNShared& s, int R, int nbits, ui
 
 constexpr int kTabBits = 10;  // k offsets for the low k bits come from LDS tables
 
+// Address maps from the output's bits: bit p of a C offset belongs to an index
+// that sits at bit mapA[p] of A (h and m roles) and/or mapB[p] of B (h and n
+// roles), -1 where it is not there.  Built per workgroup in LDS.
+__device__ __forceinline__ void c_bit_maps(const TNShared& d, signed char* mapA, signed char* mapB) {
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2];
+  for (int p = threadIdx.x; p < 32; p += blockDim.x) {
+    mapA[p] = -1;
+    mapB[p] = -1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < nH; ++j) {
+      mapA[d.pos[CH][j]] = d.pos[AH][j];
+      mapB[d.pos[CH][j]] = d.pos[BH][j];
+    }
+    for (int j = 0; j < nM; ++j) mapA[d.pos[CM][j]] = d.pos[AM][j];
+    for (int j = 0; j < nN; ++j) mapB[d.pos[CN][j]] = d.pos[BN][j];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t via_map(const signed char* map, int bits, uint32_t o) {
+  uint32_t off = 0;
+  for (int p = 0; p < bits; ++p)
+    if (map[p] >= 0) off |= ((o >> p) & 1u) << map[p];
+  return off;
+}
+
+// Output elements are enumerated in C's own address order (consecutive
+// threads write consecutive addresses); the operand offsets follow through
+// the bit maps above.
 __global__ void __launch_bounds__(256) k_tn_small(const float2* __restrict__ A, const float2* __restrict__ B,
                                                   float2* __restrict__ C, float2* __restrict__ part,
                                                   const int32_t* __restrict__ dd, int64_t a_bs, int64_t b_bs,
@@ -70,8 +104,11 @@ __global__ void __launch_bounds__(256) k_tn_small(const float2* __restrict__ A, 
   __shared__ uint32_t tka[1 << kTabBits];
   __shared__ uint32_t tkb[1 << kTabBits];
   __shared__ TNShared d;
+  __shared__ signed char mapA[32], mapB[32];
   load_desc(d, dd);
+  c_bit_maps(d, mapA, mapB);
   const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  const int rc = nH + nM + nN;
   const int tb = nK < kTabBits ? nK : kTabBits;
   for (int i = threadIdx.x; i < (1 << tb); i += blockDim.x) {
     tka[i] = scat(d, AK, tb, i);
@@ -83,15 +120,12 @@ __global__ void __launch_bounds__(256) k_tn_small(const float2* __restrict__ A, 
   const int lpo = 1 << lpo_log;
   const int sub = (int)(g & (lpo - 1));
   const int64_t rest = g >> lpo_log;
-  const int64_t o = rest >> split_log;          // output element
+  const int64_t o = rest >> split_log;                  // output element = C offset
   const int s = (int)(rest & ((1 << split_log) - 1));  // k segment
   const bool valid = o < n_out;
   const uint32_t oo = valid ? (uint32_t)o : 0u;
-  const uint32_t n = oo & ((1u << nN) - 1);
-  const uint32_t m = (oo >> nN) & ((1u << nM) - 1);
-  const uint32_t h = oo >> (nN + nM);
-  const float2* a = A + b * a_bs + (scat(d, AH, nH, h) | scat(d, AM, nM, m));
-  const float2* bp = B + b * b_bs + (scat(d, BH, nH, h) | scat(d, BN, nN, n));
+  const float2* a = A + b * a_bs + via_map(mapA, rc, oo);
+  const float2* bp = B + b * b_bs + via_map(mapB, rc, oo);
   const int64_t kcount = 1ll << nK;
   const int64_t seg = kcount >> split_log;
   const int64_t k0 = (int64_t)s * seg;
@@ -125,24 +159,79 @@ __global__ void __launch_bounds__(256) k_tn_small(const float2* __restrict__ A, 
   }
   if (!valid || sub != 0) return;
   if (split_log == 0) {
-    C[b * c_bs + (scat(d, CH, nH, h) | scat(d, CM, nM, m) | scat(d, CN, nN, n))] = acc;
+    C[b * c_bs + o] = acc;
   } else {
     part[(b << split_log | s) * n_out + o] = acc;
   }
 }
 
-__global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restrict__ C,
-                               const int32_t* __restrict__ dd, int64_t c_bs, int split_log, int64_t n_out) {
+// Streaming form for the common skinny step (a large tensor times a gate:
+// at most 16 summed terms, at least 2^12 outputs).  Workgroup g owns C
+// offsets g*4096 .. +4095; thread t handles o = g*4096 + j*256 + t for
+// j < 16, so the operand offsets split into a per-thread part (bits 0..7),
+// a per-j part (bits 8..11, 16-entry LDS tables) and a per-workgroup part
+// (bits >= 12): two ORs per element instead of a loop over the output bits.
+__global__ void __launch_bounds__(256) k_tn_stream(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                   float2* __restrict__ C, const int32_t* __restrict__ dd,
+                                                   int64_t a_bs, int64_t b_bs, int64_t c_bs) {
   __shared__ TNShared d;
+  __shared__ signed char mapA[32], mapB[32];
+  __shared__ uint32_t midA[16], midB[16], tka[16], tkb[16];
   load_desc(d, dd);
+  c_bit_maps(d, mapA, mapB);
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  const int rc = nH + nM + nN;
+  const int t = threadIdx.x;
+  if (t < 16) {
+    uint32_t ma = 0, mb = 0;
+    for (int p = 8; p < 12; ++p) {
+      const uint32_t bit = (t >> (p - 8)) & 1u;
+      if (mapA[p] >= 0) ma |= bit << mapA[p];
+      if (mapB[p] >= 0) mb |= bit << mapB[p];
+    }
+    midA[t] = ma;
+    midB[t] = mb;
+  } else if (t >= 32 && t < 32 + (1 << nK)) {
+    tka[t - 32] = scat(d, AK, nK, t - 32);
+    tkb[t - 32] = scat(d, BK, nK, t - 32);
+  }
+  __syncthreads();
+  const uint32_t g = blockIdx.x;
+  uint32_t hiA = 0, hiB = 0, loA = 0, loB = 0;
+  for (int p = 12; p < rc; ++p) {
+    const uint32_t bit = (g >> (p - 12)) & 1u;
+    if (mapA[p] >= 0) hiA |= bit << mapA[p];
+    if (mapB[p] >= 0) hiB |= bit << mapB[p];
+  }
+  for (int p = 0; p < 8; ++p) {
+    const uint32_t bit = ((uint32_t)t >> p) & 1u;
+    if (mapA[p] >= 0) loA |= bit << mapA[p];
+    if (mapB[p] >= 0) loB |= bit << mapB[p];
+  }
+  const int64_t b = blockIdx.y;
+  const float2* a = A + b * a_bs + (hiA | loA);
+  const float2* bp = B + b * b_bs + (hiB | loB);
+  float2* c = C + b * c_bs + ((int64_t)g << 12) + t;
+  const int kc = 1 << nK;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const uint32_t oa = midA[j], ob = midB[j];
+    float2 acc = make_float2(0.f, 0.f);
+    for (int k = 0; k < kc; ++k) {
+      const float2 x = a[oa | tka[k]];
+      const float2 y = bp[ob | tkb[k]];
+      acc.x = fmaf(x.x, y.x, fmaf(-x.y, y.y, acc.x));
+      acc.y = fmaf(x.x, y.y, fmaf(x.y, y.x, acc.y));
+    }
+    c[j << 8] = acc;
+  }
+}
+
+__global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restrict__ C, int64_t c_bs, int split_log,
+                               int64_t n_out) {
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= n_out) return;
   const int64_t b = blockIdx.y;
-  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2];
-  const uint32_t oo = (uint32_t)o;
-  const uint32_t n = oo & ((1u << nN) - 1);
-  const uint32_t m = (oo >> nN) & ((1u << nM) - 1);
-  const uint32_t h = oo >> (nN + nM);
   float2 acc = make_float2(0.f, 0.f);
   const int S = 1 << split_log;
   for (int s = 0; s < S; ++s) {
@@ -150,7 +239,7 @@ __global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restri
     acc.x += v.x;
     acc.y += v.y;
   }
-  C[b * c_bs + (scat(d, CH, nH, h) | scat(d, CM, nM, m) | scat(d, CN, nN, n))] = acc;
+  C[b * c_bs + o] = acc;
 }
 
 // 64x64 output tile of one (batch, h); k staged 16 at a time through LDS; MFMA.
@@ -270,6 +359,7 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
 // ---------------------------------------------------------------------------
 struct TNLaunch {
   bool gemm;
+  bool stream;
   int lpo_log, split_log;
   int64_t n_out;
 };
@@ -309,9 +399,10 @@ static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
   const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
   L.n_out = 1ll << (nH + nM + nN);
   L.gemm = nM >= 6 && nN >= 6 && nK >= 4;
+  L.stream = !L.gemm && nK <= 4 && nH + nM + nN >= 12;
   L.lpo_log = 0;
   L.split_log = 0;
-  if (L.gemm) return L;
+  if (L.gemm || L.stream) return L;
   // aim for >= 2^19 threads in flight and <= 2^12 k iterations per thread
   const int64_t want = 1ll << 19;
   int64_t threads = L.n_out * batch;
@@ -373,6 +464,16 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
                        a_bstride, b_bstride, c_bstride, m_tiles);
     return check_launch("tedq_tn_contract(gemm)");
   }
+  if (L.stream) {
+    const int64_t blocks = L.n_out >> 12;
+    if (blocks > 0x7fffffffll) {
+      set_error("tedq_tn_contract: grid too large (%lld blocks)", (long long)blocks);
+      return kBadArgument;
+    }
+    hipLaunchKernelGGL(k_tn_stream, dim3((unsigned)blocks, (unsigned)batch), dim3(256), 0, st, A, Bp, Cp, desc_dev,
+                       a_bstride, b_bstride, c_bstride);
+    return check_launch("tedq_tn_contract(stream)");
+  }
   const int64_t need = L.split_log ? (int64_t)sizeof(float2) * batch * (L.n_out << L.split_log) : 0;
   if (need > 0 && (workspace == nullptr || workspace_bytes < need)) {
     set_error("tedq_tn_contract: workspace of %lld bytes required, %lld given", (long long)need,
@@ -391,7 +492,7 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
   int rc = check_launch("tedq_tn_contract(small)");
   if (rc != kOk || L.split_log == 0) return rc;
   const int64_t sb = (L.n_out + 255) / 256;
-  hipLaunchKernelGGL(k_tn_split_sum, dim3((unsigned)sb, (unsigned)batch), dim3(256), 0, st, part, Cp, desc_dev,
-                     c_bstride, L.split_log, L.n_out);
+  hipLaunchKernelGGL(k_tn_split_sum, dim3((unsigned)sb, (unsigned)batch), dim3(256), 0, st, part, Cp, c_bstride,
+                     L.split_log, L.n_out);
   return check_launch("tedq_tn_contract(split sum)");
 }

@@ -110,6 +110,22 @@ def make_desc(la, lb, lc):
     return d
 
 
+def _step_layout(la, lb, res):
+    """Index order of an intermediate.  When a small tensor (a gate) meets a large one, the result keeps
+    the large tensor's layout with each summed index replaced by the matching new index of the small one
+    (wire i of a gate's [new..., old...] pairs new[i] with old[i]), so the step reads and writes the large
+    tensor at the same addresses; otherwise A's kept indices, then B's."""
+    big, small = (la, lb) if len(la) >= len(lb) else (lb, la)
+    if len(small) <= 8:
+        sbig = set(big)
+        summed = [x for x in small if x in sbig and x not in res]
+        new = [x for x in small if x in res and x not in sbig]
+        if summed and len(summed) == len(new):
+            mp = dict(zip(summed, new))
+            return [mp.get(x, x) for x in big]
+    return [x for x in la if x in res] + [x for x in lb if x in res and x not in la]
+
+
 @dataclass
 class _Step:
     a: int
@@ -149,8 +165,7 @@ class _Network:
                 if set(out) != set(res):
                     raise ValueError("tensor network: final tensor does not carry the output indices")
             else:
-                la, lb = lay[i], lay[j]
-                out = [x for x in la if x in res] + [x for x in lb if x in res and x not in la]
+                out = _step_layout(lay[i], lay[j], res)
             lay.append(out)
             needs.append(needs[i] or needs[j])
             fwd = desc_rows(lay[i], lay[j], out)
