@@ -299,6 +299,7 @@ def main():
     ap.add_argument("--skip-tn", action="store_true")
     ap.add_argument("--no-jit", action="store_true", help="keep the interpreted pass kernels")
     ap.add_argument("--skip-c4", action="store_true")
+    ap.add_argument("--tn-only", action="store_true", help="run only the contraction-mode leg and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for traffic)")
@@ -317,6 +318,9 @@ def main():
         sys.exit(2)
 
     import tedq_amd  # noqa: F401  pylint: disable=import-outside-toplevel,unused-import
+    if args.tn_only:
+        print(json.dumps({"tensor_network": tn_leg(c4=not args.skip_c4)}), flush=True)
+        return
     dev = torch.device("cuda", 0)
     n, L, B = args.qubits, args.layers, args.batch
     rng = np.random.default_rng(1)

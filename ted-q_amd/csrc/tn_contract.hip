@@ -399,7 +399,8 @@ static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
   const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
   L.n_out = 1ll << (nH + nM + nN);
   L.gemm = nM >= 6 && nN >= 6 && nK >= 4;
-  L.stream = !L.gemm && nK <= 4 && nH + nM + nN >= 12;
+  // a large tensor times a gate-sized one (the host keeps the large tensor's layout for such steps)
+  L.stream = !L.gemm && nK <= 4 && nH + nM + nN >= 12 && (nH + nK + nN <= 8 || nH + nM + nK <= 8);
   L.lpo_log = 0;
   L.split_log = 0;
   if (L.gemm || L.stream) return L;
