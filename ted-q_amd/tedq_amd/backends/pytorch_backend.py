@@ -127,6 +127,8 @@ class PyTorchBackend(CompiledCircuit):
         self._tile_bits_bwd = kwargs.get("tile_bits_bwd", None)
         # compile per-pass kernels for this circuit structure at first use (hiprtc, a few seconds)
         self._jit = bool(kwargs.get("jit", False))
+        # contraction mode: externally found paths, one per measurement (SSA or opt_einsum-style linear)
+        self._tn_paths = kwargs.get("tn_paths", None)
         self._reg_bits_bwd = kwargs.get("reg_bits_bwd", None)
         if self._interface != "pytorch":
             raise ValueError(f"{self._interface}: pytroch_backend only supports pytorch interface!")
@@ -195,7 +197,7 @@ class PyTorchBackend(CompiledCircuit):
         if self._executor is None:
             if self._tn_mode or self._use_cotengra or self._use_jdopttn:
                 from ..tn.executor import TNExecutor  # pylint: disable=import-outside-toplevel
-                self._executor = TNExecutor(self)
+                self._executor = TNExecutor(self, paths=self._tn_paths)
             else:
                 n, units, n_params, pool = lower_circuit(self)
                 init = None

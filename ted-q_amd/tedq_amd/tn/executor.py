@@ -38,7 +38,7 @@ from ..engine import hip
 from ..engine.sv import MeasSpec
 from ..measurement import Expectation, Probability, State
 from .network import cancel_inverse_pairs
-from .path import search_path
+from .path import external_path, path_sets, path_stats, search_path
 
 _BLOCK_CODE = {"RX": 14, "RY": 15, "RZ": 16, "Rot": 17, "PhaseShift": 18,
                "CRX": 14, "CRY": 15, "CRZ": 16, "ControlledPhaseShift": 18}
@@ -139,19 +139,29 @@ class _Step:
 class _Network:
     """One reduced network with its frozen path and index layouts."""
 
-    def __init__(self, tn, leaf_src, desc_rows, trials):
-        red = cancel_inverse_pairs(tn)
+    def __init__(self, tn, leaf_src, desc_rows, trials, external=None):
+        # an externally found order applies to the network as built (no light-cone reduction)
+        red = tn if external is not None else cancel_inverse_pairs(tn)
         inputs = [list(t) for t in red.input_indices]
         sources = list(red.sources)
         if len(inputs) < 2:                       # keep at least one step
             inputs.append([])
             sources.append(("one",))
+            if external is not None:
+                external = [(0, 1)]
         self.measurement = tn.measurement
         self.output = list(tn.output_indices)
         self.n_inputs = len(inputs)
         self.sources = sources
         self.leaves = [leaf_src(s) for s in sources]
-        path, sets, info = search_path(inputs, self.output, trials)
+        if external is not None:
+            path = [tuple(p) for p in external]
+            sets = path_sets(inputs, self.output, path)
+            largest, _ = path_stats(path, sets, len(inputs))
+            info = {"strategy": "external", "trial": None,
+                    "macs": sum(1 << len(sets[i] | sets[j]) for i, j in path), "largest_rank": largest}
+        else:
+            path, sets, info = search_path(inputs, self.output, trials)
         self.path_info = info
         self.path = path
         lay = [list(t) for t in inputs]
@@ -186,6 +196,36 @@ class _Network:
         return sum(1 << self.ranks[st.c] for st in self.steps)
 
 
+def _external_paths(cc, paths):
+    """Per network: an SSA path from ``paths`` (SSA or linear, one per measurement), or from the
+    optimiser object the circuit was compiled with (the reference's ``use_cotengra`` / ``use_jdopttn``
+    calls, tedq/backends/compiled_circuit.py:340-393), or None (own search)."""
+    # pylint: disable=protected-access
+    nets = cc._tensor_networks
+    if paths is not None:
+        if len(paths) != len(nets):
+            raise ValueError(f"{len(paths)} contraction paths given for {len(nets)} networks")
+        return [external_path(p, tn.num_tensors) for p, tn in zip(paths, nets)]
+    hyper = cc._hyper_opt if isinstance(cc._hyper_opt, dict) else {}
+    ctg, jd = cc._use_cotengra, cc._use_jdopttn
+    out = []
+    for tn in nets:
+        tree = None
+        if ctg is not None and hasattr(ctg, "HyperOptimizer"):
+            opt = ctg.HyperOptimizer(methods=hyper.get("methods", ["kahypar"]),
+                                     max_repeats=hyper.get("max_repeats", 128),
+                                     progbar=hyper.get("progbar", False), minimize=hyper.get("minimize", "flops"),
+                                     score_compression=hyper.get("score_compression", 0.5),
+                                     slicing_opts=hyper.get("slicing_opts", None))
+            tree = opt.search(tn.input_indices, tn.output_indices, tn.size_dict)
+        elif callable(jd) and not isinstance(jd, bool):
+            tree = jd(tn.input_indices, tn.size_dict, output=tn.output_indices, imbalance=0.2,
+                      max_repeats=hyper.get("max_repeats", 128), search_parallel=hyper.get("search_parallel", True),
+                      slicing_opts=hyper.get("slicing_opts", None))
+        out.append(None if tree is None else external_path(tree, tn.num_tensors))
+    return out
+
+
 class TNExecutor:
     """Contraction-mode executor of one compiled circuit (all its measurements).
 
@@ -194,7 +234,7 @@ class TNExecutor:
     ``specs`` and ``n_params``.
     """
 
-    def __init__(self, cc, trials=None, max_bytes=64 << 30):
+    def __init__(self, cc, trials=None, max_bytes=64 << 30, paths=None):
         from .path import SEARCH_TRIALS  # pylint: disable=import-outside-toplevel
         trials = SEARCH_TRIALS if trials is None else trials
         self.n = cc._num_qubits  # pylint: disable=protected-access
@@ -261,6 +301,7 @@ class TNExecutor:
 
         self.networks = []
         self.specs = []
+        ext = _external_paths(cc, paths)
         for tn in cc._tensor_networks:  # pylint: disable=protected-access
             meas = tn.measurement
 
@@ -282,7 +323,7 @@ class TNExecutor:
                 self._const_len += m.size
                 return ("const", o)
 
-            net = _Network(tn, leaf_src, desc_rows, trials)
+            net = _Network(tn, leaf_src, desc_rows, trials, ext[len(self.networks)])
             self.networks.append(net)
             if meas.return_type is Expectation:
                 self.specs.append(MeasSpec("fexp", ()))

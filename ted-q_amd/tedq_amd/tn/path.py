@@ -175,3 +175,62 @@ def path_stats(path, sets, n_inputs):
         largest = max(largest, len(res))
         flops += 1 << len(sets[i] | sets[j])
     return largest, math.log10(max(flops, 1))
+
+
+def linear_to_ssa(path, n_inputs):
+    """opt_einsum-style path (positions in the shrinking operand list, result appended) -> SSA path."""
+    ids = list(range(n_inputs))
+    nxt = n_inputs
+    ssa = []
+    for step in path:
+        if len(step) != 2:
+            raise ValueError("only pairwise contraction steps are supported")
+        i, j = sorted(step)
+        a, b = ids[i], ids[j]
+        del ids[j]
+        del ids[i]
+        ids.append(nxt)
+        ssa.append((min(a, b), max(a, b)))
+        nxt += 1
+    return ssa
+
+
+def path_sets(inputs, output, ssa):
+    """Index sets of every tensor along an SSA path (validated: each tensor used exactly once)."""
+    out_set = set(output)
+    sets = [frozenset(t) for t in inputs]
+    counts = {}
+    for t in sets:
+        for ix in t:
+            counts[ix] = counts.get(ix, 0) + 1
+    used = set()
+    for i, j in ssa:
+        if i in used or j in used or i == j or max(i, j) >= len(sets):
+            raise ValueError(f"contraction path step ({i}, {j}) is not a valid SSA step")
+        used |= {i, j}
+        _, res = _cost(sets[i], sets[j], counts, out_set)
+        for ix in sets[i] | sets[j]:
+            counts[ix] -= (ix in sets[i]) + (ix in sets[j])
+        for ix in res:
+            counts[ix] += 1
+        sets.append(res)
+    if len(ssa) != max(len(inputs) - 1, 0) or (ssa and len(used) != len(sets) - 1):
+        raise ValueError("contraction path does not contract every tensor into one")
+    return sets
+
+
+def external_path(tree, n_inputs):
+    """SSA path from a contraction tree or path object of an external optimiser.
+
+    Accepts what cotengra's ``HyperOptimizer.search`` and JDtensorPath return
+    (objects with ``get_ssa_path()`` or ``get_path()``), opt_einsum-style
+    objects with a ``path`` attribute (linear), or a plain list of pairs
+    (linear).  Sliced trees are contracted whole.
+    """
+    if hasattr(tree, "get_ssa_path"):
+        return [tuple(int(x) for x in st) for st in tree.get_ssa_path()]
+    if hasattr(tree, "get_path"):
+        return linear_to_ssa([tuple(st) for st in tree.get_path()], n_inputs)
+    if hasattr(tree, "path"):
+        return linear_to_ssa([tuple(st) for st in tree.path], n_inputs)
+    return linear_to_ssa([tuple(st) for st in tree], n_inputs)

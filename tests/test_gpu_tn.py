@@ -190,3 +190,22 @@ def test_compiled_call_autograd_in_contraction_mode():
     ref.sum().backward()
     assert np.max(np.abs(out.detach().cpu().numpy() - ref.detach().numpy())) < 2e-5
     assert np.max(np.abs(p.grad.cpu().numpy() - th.grad.numpy())) < 1e-4
+
+
+def test_external_linear_path_on_the_full_network():
+    """Paths found elsewhere (opt_einsum-style linear, one per measurement) drive the contraction."""
+    n, steps, obs = 8, 2, 3
+    cc0 = W.disordered_chain(n, steps, obs, tn_mode=True)
+    nt = cc0._tensor_networks[0].num_tensors  # pylint: disable=protected-access
+    cc = W.disordered_chain(n, steps, obs, tn_mode=True, tn_paths=[[(0, 1)] * (nt - 1)])
+    ex = cc.executor()
+    assert ex.path_report()[0]["strategy"] == "external"
+    rng = np.random.default_rng(8)
+    rows = torch.tensor(W.chain_rows(rng.uniform(-5, 5, size=(3, n)), steps), device=DEV)
+    outs, grad = ex.value_and_grad(rows, [torch.ones(3, device=DEV)])
+    for b in range(3):
+        th = torch.tensor(rows[b].cpu().numpy(), dtype=torch.float64, requires_grad=True)
+        ref = O.execute(n, _chain_gates(n, steps, list(th)), [("expval", ("PauliZ", [obs]))])
+        ref.sum().backward()
+        assert abs(float(outs[0][b]) - float(ref.detach())) < 2e-5
+        assert np.max(np.abs(grad[b].cpu().numpy() - th.grad.numpy())) < 1e-4
