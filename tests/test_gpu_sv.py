@@ -324,3 +324,31 @@ def test_generated_kernels_on_the_benchmark_circuit():
     vj, gj = ex_j.value_and_grad(rows, ups)
     assert np.max(np.abs(torch.stack(vi).cpu().numpy() - torch.stack(vj).cpu().numpy())) < 1e-6
     assert np.max(np.abs(gi.cpu().numpy() - gj.cpu().numpy())) < 1e-5
+
+
+def test_large_states_beyond_buffer_addressing():
+    """n = 29 (4 GiB per state, past the 2^28-amplitude buffer-resource path): closed forms.
+    RY(a) on qubit 0, RX(b) on qubit 28, then CNOT(0,1) ... CNOT(27,28): qubits 1..27 copy
+    qubit 0, qubit 28 becomes its initial bit XOR qubit 0, so <Z_14> = cos a and
+    <Z_28> = cos a cos b, with the matching derivatives."""
+    n = 29
+
+    def cd(a, b):
+        qai.RY(a, qubits=[0])
+        qai.RX(b, qubits=[n - 1])
+        for q in range(n - 1):
+            qai.CNOT(qubits=[q, q + 1])
+        return [qai.expval(qai.PauliZ(qubits=[14])), qai.expval(qai.PauliZ(qubits=[n - 1]))]
+
+    a0, b0 = 0.7, -1.3
+    c = qai.Circuit(cd, n, torch.tensor(a0), torch.tensor(b0)).compilecircuit(backend="pytorch")
+    a = torch.tensor([a0], device=DEV, requires_grad=True)
+    b = torch.tensor([b0], device=DEV, requires_grad=True)
+    out = c(a, b)
+    out[1].backward()
+    assert abs(float(out[0]) - math.cos(a0)) < 1e-5
+    assert abs(float(out[1]) - math.cos(a0) * math.cos(b0)) < 1e-5
+    assert abs(float(a.grad) + math.sin(a0) * math.cos(b0)) < 1e-4
+    assert abs(float(b.grad) + math.cos(a0) * math.sin(b0)) < 1e-4
+    del out
+    torch.cuda.empty_cache()
