@@ -352,3 +352,32 @@ def test_large_states_beyond_buffer_addressing():
     assert abs(float(b.grad) + math.cos(a0) * math.sin(b0)) < 1e-4
     del out
     torch.cuda.empty_cache()
+
+
+def test_notebook_workflow_values_and_mse_gradient():
+    """The reference example notebook, written as there (measurements recorded without a return,
+    params one tensor, MSE loss), on the GPU: printed values and gradient to 4 decimals."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_values.json")))
+    case = g["notebook_expval_and_gradient"]
+    n_qubits = 3
+
+    def circuitDef(theta):  # pylint: disable=invalid-name
+        qai.RX(theta[0], qubits=[0])
+        qai.RY(theta[1], qubits=[1])
+        qai.RZ(theta[2], qubits=[2])
+        qai.CNOT(qubits=[0, 1])
+        qai.CNOT(qubits=[1, 2])
+        qai.CNOT(qubits=[2, 0])
+        for idx in range(n_qubits):
+            qai.measurement.expval(qai.PauliZ(qubits=[idx]))
+
+    circuit = qai.Circuit(circuitDef, n_qubits, torch.rand(n_qubits))
+    compiled = circuit.compilecircuit(backend="pytorch")
+    params = torch.tensor(case["params_printed_4dp"], device=DEV, requires_grad=True)
+    exp_val = compiled(params)
+    loss = torch.nn.MSELoss(reduction="mean")(exp_val, torch.tensor(case["mse_target"], device=DEV))
+    loss.backward()
+    assert np.max(np.abs(exp_val.detach().cpu().numpy() - case["expval_printed_4dp"])) < 5e-4
+    assert np.max(np.abs(params.grad.cpu().numpy() - case["grad_of_mse_printed_4dp"])) < 5e-4

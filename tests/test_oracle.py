@@ -124,3 +124,20 @@ def test_gradient_identities():
         return [("Hadamard", [0], []), ("CRX", [0, 1], [th[0]])], ["CRX"]
     m2 = [("expval", ("PauliY", [1]))]
     assert np.allclose(O.param_shift_jacobian(2, crx, ct, m2), O.finite_diff_jacobian(2, crx, ct, m2), atol=1e-7)
+
+
+def test_golden_notebook_values_and_mse_gradient():
+    """examples/Expected_value_and_the_gradient_of_parameters.ipynb: values and MSE gradient as printed
+    (4 decimals; the printed parameters are themselves rounded, hence the 5e-4 tolerance)."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_values.json")))
+    case = g["notebook_expval_and_gradient"]
+    t = torch.tensor(case["params_printed_4dp"], dtype=torch.float64, requires_grad=True)
+    gates = [("RX", [0], [t[0]]), ("RY", [1], [t[1]]), ("RZ", [2], [t[2]]),
+             ("CNOT", [0, 1], []), ("CNOT", [1, 2], []), ("CNOT", [2, 0], [])]
+    ev = O.execute(3, gates, [("expval", ("PauliZ", [q])) for q in range(3)])
+    assert np.max(np.abs(ev.detach().numpy() - case["expval_printed_4dp"])) < 5e-4
+    loss = torch.nn.functional.mse_loss(ev, torch.tensor(case["mse_target"], dtype=torch.float64))
+    loss.backward()
+    assert np.max(np.abs(t.grad.numpy() - case["grad_of_mse_printed_4dp"])) < 5e-4
