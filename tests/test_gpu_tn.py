@@ -209,3 +209,52 @@ def test_external_linear_path_on_the_full_network():
         ref.sum().backward()
         assert abs(float(outs[0][b]) - float(ref.detach())) < 2e-5
         assert np.max(np.abs(grad[b].cpu().numpy() - th.grad.numpy())) < 1e-4
+
+
+@pytest.mark.parametrize("kind", ["probs", "state", "hermitian"])
+def test_probs_state_and_matrix_observables(kind):
+    """Contraction mode for the other measurement kinds: probabilities (ket/bra diagonal outputs),
+    the state (ket only, complex output), and a user Hermitian matrix; values and gradients."""
+    n = 5
+    herm = np.array([[0.3, 0.2 - 0.5j], [0.2 + 0.5j, -0.8]])
+
+    def cd(*p):
+        qai.RX(p[0], qubits=[0])
+        qai.CRY(p[1], qubits=[0, 3])
+        qai.Rot(p[2], p[3], p[4], qubits=[2])
+        qai.CNOT(qubits=[2, 4])
+        qai.CZ(qubits=[4, 1])
+        qai.RY(p[5], qubits=[1])
+        if kind == "probs":
+            return qai.probs(qubits=[1, 3])
+        if kind == "state":
+            return qai.state()
+        return qai.expval(qai.Unitary(herm, qubits=[4]))
+
+    theta = np.array([0.4, -1.1, 0.9, 2.2, -0.6, 1.3])
+    cc = qai.Circuit(cd, n, *([0.1] * 6)).compilecircuit(backend="pytorch", tn_mode=True)
+    p = torch.tensor(theta, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = cc(p)
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    gates = [("RX", [0], [th[0]]), ("CRY", [0, 3], [th[1]]), ("Rot", [2], [th[2], th[3], th[4]]),
+             ("CNOT", [2, 4], []), ("CZ", [4, 1], []), ("RY", [1], [th[5]])]
+    if kind == "probs":
+        ref = O.execute(n, gates, [("probs", [1, 3])])
+        w = torch.arange(4, dtype=torch.float64).reshape(1, 2, 2) + 1.0
+    elif kind == "state":
+        ref = O.execute(n, gates, [("state",)])
+        w = None
+    else:
+        ref = O.execute(n, gates, [("expval", (herm, [4]))])
+        w = torch.ones(1, dtype=torch.float64)
+    got = out.detach().cpu().numpy()
+    assert np.max(np.abs(got - ref.detach().numpy())) < 2e-5
+    if kind == "state":
+        # a real loss of the complex output: sum of Re(psi) weighted + |psi_3|^2
+        lw = torch.linspace(-1, 1, 2 ** n, dtype=torch.float64).reshape(out.shape)
+        (out.real * lw.to(DEV, torch.float32)).sum().backward()
+        (ref.real * lw).sum().backward()
+    else:
+        (out * w.to(DEV, torch.float32)).sum().backward()
+        (ref * w).sum().backward()
+    assert np.max(np.abs(p.grad.cpu().numpy() - th.grad.numpy())) < 1e-4
