@@ -29,6 +29,8 @@
 //     complex block and k-step); used when m, n and k are all wide.
 // Arithmetic is complex64 with fp32 accumulation, like the reference's
 // complex64 torch contraction.
+#include <stdlib.h>
+
 #include "tedq_common.h"
 
 namespace tedq {
@@ -247,6 +249,7 @@ constexpr int kTile = 64;
 constexpr int kChunk = 16;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+template <bool THREE_M>
 __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, const float2* __restrict__ B,
                                                  float2* __restrict__ C, const int32_t* __restrict__ dd,
                                                  int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles) {
@@ -282,13 +285,15 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
   const int lane = t & 63, w = t >> 6;
   const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  f32x4 cr[2][2], ci[2][2];
+  // 4M: cr, ci are the real and imaginary parts; 3M: cr, ci, c3 hold p1, p2, p3
+  f32x4 cr[2][2], ci[2][2], c3[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       cr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       ci[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      c3[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
   const uint32_t ksz = 1u << nK;
   for (uint32_t k0 = 0; k0 < ksz; k0 += kChunk) {
@@ -320,24 +325,45 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
       for (int i = 0; i < 2; ++i) x[i] = As[ks + lk][wm + 16 * i + lr];
 #pragma unroll
       for (int j = 0; j < 2; ++j) y[j] = Bs[ks + lk][wn + 16 * j + lr];
-      // one term at a time over the four blocks, so that consecutive MFMAs
-      // never wait on each other's accumulator
+      if (THREE_M) {
+        // 3M: p1 += ar br, p2 += ai bi, p3 += (ar + ai)(br + bi); re = p1 - p2, im = p3 - p1 - p2
+        float xs[2], ys[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i) xs[i] = x[i].x + x[i].y;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].x, cr[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j) ys[j] = y[j].x + y[j].y;
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].y, ci[i][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].x, cr[i][j], 0, 0, 0);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-x[i].y, y[j].y, cr[i][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].y, y[j].y, ci[i][j], 0, 0, 0);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].y, y[j].x, ci[i][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j) c3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xs[i], ys[j], c3[i][j], 0, 0, 0);
+      } else {
+        // 4M, one term at a time over the four blocks, so that consecutive
+        // MFMAs never wait on each other's accumulator
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].x, cr[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].y, ci[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-x[i].y, y[j].y, cr[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].y, y[j].x, ci[i][j], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
@@ -350,7 +376,11 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
       for (int v = 0; v < 4; ++v) {
         const uint32_t mi = wm + 16 * i + 4 * lk + v;
         const uint32_t ni = wn + 16 * j + lr;
-        if (m0 + mi < msz && n0 + ni < nsz) c[rowC[mi] | colC[ni]] = make_float2(cr[i][j][v], ci[i][j][v]);
+        if (m0 + mi < msz && n0 + ni < nsz) {
+          const float re = THREE_M ? cr[i][j][v] - ci[i][j][v] : cr[i][j][v];
+          const float im = THREE_M ? c3[i][j][v] - cr[i][j][v] - ci[i][j][v] : ci[i][j][v];
+          c[rowC[mi] | colC[ni]] = make_float2(re, im);
+        }
       }
 }
 
@@ -461,8 +491,19 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
       set_error("tedq_tn_contract: grid too large (%lld x %lld)", (long long)gx, (long long)gy);
       return kBadArgument;
     }
-    hipLaunchKernelGGL(k_tn_gemm, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp, Cp, desc_dev,
-                       a_bstride, b_bstride, c_bstride, m_tiles);
+    // complex products as 4 real MFMA products by default; TEDQ_TN_GEMM=3m selects the 3-product
+    // (Gauss) form, which issues 25% fewer MFMAs but needs a third accumulator set (132 VGPRs
+    // against 90) and measured slower on the 40-qubit chain (191 ms against 170 ms per batch)
+    static const bool four_m = [] {
+      const char* e = getenv("TEDQ_TN_GEMM");
+      return e == nullptr || e[0] != '3';
+    }();
+    if (four_m)
+      hipLaunchKernelGGL(k_tn_gemm<false>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp,
+                         Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
+    else
+      hipLaunchKernelGGL(k_tn_gemm<true>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp,
+                         Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
     return check_launch("tedq_tn_contract(gemm)");
   }
   if (L.stream) {
