@@ -35,6 +35,12 @@ import numpy as np  # noqa: E402
 import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """One line on stderr per finished phase (the JSON result alone goes to stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 class EventProbe:
@@ -108,28 +114,38 @@ def plain_eval(n, layers, theta_row, device):
 
 
 def time_plain_gpu(n, layers, theta, device, budget_s=8.0):
+    """Rows per second of plain_eval + autograd backward, one row per call.
+
+    Warmed with one single-layer call (library handles and every kernel shape
+    the full call uses), then timed over whole calls until ``budget_s`` has
+    passed.  Returns (evals/s, calls, forward s, backward s) of the timed calls."""
     rows = torch.tensor(theta, dtype=torch.float32, device=device)
-    done, t0 = 0, None
+    r = rows[0, :2 * n].clone().requires_grad_(True)
+    plain_eval(n, 1, r, device).sum().backward()
+    torch.cuda.synchronize()
+    done, fwd_s, bwd_s = 0, 0.0, 0.0
+    t0 = time.perf_counter()
     for i in range(rows.shape[0]):
         r = rows[i].clone().requires_grad_(True)
-        if i == 1:
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
+        ta = time.perf_counter()
         out = plain_eval(n, layers, r, device)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
         out.sum().backward()
-        if i >= 1:
-            done += 1
-            torch.cuda.synchronize()
-            if time.perf_counter() - t0 > budget_s:
-                break
-    torch.cuda.synchronize()
-    return done / (time.perf_counter() - t0), done
+        torch.cuda.synchronize()
+        fwd_s += tb - ta
+        bwd_s += time.perf_counter() - tb
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    return done / (time.perf_counter() - t0), done, fwd_s / done, bwd_s / done
 
 
 def cpu_baseline(n, layers, theta, budget_s=15.0):
     """The oracle's complex64 restatement on the host cores (bounded sample)."""
     from oracle import sv_oracle as O  # pylint: disable=import-outside-toplevel
-    cores = os.cpu_count() or 1
+    # the host threads this process may use: the OMP_NUM_THREADS share when set, else its CPU affinity
+    cores = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
     meas = [("expval", ("PauliZ", [q])) for q in range(n)]
     done = 0
@@ -193,24 +209,27 @@ def serving_leg(n_req, warm):
     eng = ServingEngine(factory, hbm_budget=64 << 30, max_batch=64, n_streams=4, max_inflight=8)
     closed = serving_stream(n_req)
     keys = sorted({r.key for r in closed} | {r.key for r in serving_stream(n_req, seed=5, rate=1.0)})
-    eng.prepare(keys, buckets=(1, 2, 4))
+    eng.prepare(keys, buckets=(1, 2, 4), log=progress)
     eng.run(serving_stream(warm, seed=99))
     eng.completed.clear()
     eng.batch_sizes.clear()
     eng.run(closed)
     cap = latency_stats(closed)
+    progress(f"serving: closed-loop capacity {cap['evals_per_s']:.0f} evals/s")
     rate = 0.7 * cap["evals_per_s"]
     eng.completed.clear()
     eng.batch_sizes.clear()
     stream = serving_stream(n_req, seed=5, rate=rate)
     eng.run(stream)
     st = latency_stats(stream)
+    progress(f"serving: Poisson stream p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
     mean_batch = float(np.mean(eng.batch_sizes))
     single = ServingEngine(factory, hbm_budget=64 << 30, max_batch=1, n_streams=1, max_inflight=1,
                            graph_max_qubits=0)
     sub = serving_stream(min(n_req, 400), seed=6)
     single.run(sub)
     un = latency_stats(sub)
+    progress(f"serving: unbatched {un['evals_per_s']:.0f} evals/s")
     return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "evals_per_s": st["evals_per_s"], "arrival_rate": rate, "requests": st["requests"],
             "capacity_evals_per_s": cap["evals_per_s"], "mean_batch": mean_batch,
@@ -252,6 +271,7 @@ def tn_leg(c4=True):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
     info = ex.path_report()[0]
+    progress(f"contraction: 40-qubit chain {dt * 1e3:.1f} ms per batch of 32")
     out["chain40"] = {"workload": "1-D disordered chain 40q, 8 Trotter steps, <Z_20>, batch 32 seeds, forward",
                       "evals_per_s": 32 / dt, "ms_per_batch": dt * 1e3, "path_build_s": build_s,
                       "steps": info["steps"], "largest_rank": info["largest_rank"], "macs_per_eval": info["macs"],
@@ -261,6 +281,7 @@ def tn_leg(c4=True):
         t0 = time.perf_counter()
         ex = W.lattice(5, 6, tn_mode=True).executor()
         build_s = time.perf_counter() - t0
+        progress(f"contraction: 5x5 lattice paths found in {build_s:.1f} s")
         rng = np.random.default_rng(3)
         rows = torch.tensor(rng.uniform(0, 2 * np.pi, size=(1, 300)), dtype=torch.float32, device=dev)
         ups = [torch.full((1,), 1.0 / 25, device=dev) for _ in range(25)]
@@ -270,6 +291,7 @@ def tn_leg(c4=True):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         rep = ex.path_report()
+        progress(f"contraction: 5x5 lattice value and gradient in {dt:.1f} s")
         out["lattice5x5"] = {"workload": "5x5 lattice 6 layers, mean <Z_i> over 25 networks, fwd + reverse-tree grad",
                              "s_per_eval": dt, "path_build_s": build_s,
                              "steps": sum(r["steps"] for r in rep),
@@ -300,6 +322,8 @@ def main():
     ap.add_argument("--no-jit", action="store_true", help="keep the interpreted pass kernels")
     ap.add_argument("--skip-c4", action="store_true")
     ap.add_argument("--tn-only", action="store_true", help="run only the contraction-mode leg and print its JSON")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="time one call of the plain per-gate torch path (--qubits, --layers) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for traffic)")
@@ -321,6 +345,14 @@ def main():
     if args.tn_only:
         print(json.dumps({"tensor_network": tn_leg(c4=not args.skip_c4)}), flush=True)
         return
+    if args.plain_only:
+        theta = np.random.default_rng(1).uniform(0, 2 * np.pi, size=(2, args.layers * args.qubits * 2))
+        ps, cnt, fwd_s, bwd_s = time_plain_gpu(args.qubits, args.layers, theta, torch.device("cuda", 0),
+                                               budget_s=0.0)
+        print(json.dumps({"plain_gpu": {"value": ps, "unit": "evals/s", "calls": cnt, "forward_s": fwd_s,
+                                        "backward_s": bwd_s, "qubits": args.qubits, "layers": args.layers}}),
+              flush=True)
+        return
     dev = torch.device("cuda", 0)
     n, L, B = args.qubits, args.layers, args.batch
     rng = np.random.default_rng(1)
@@ -334,7 +366,7 @@ def main():
     tuning = None
     if not args.no_autotune:
         from tedq_amd.engine.autotune import autotune  # pylint: disable=import-outside-toplevel
-        res = autotune(ex, rows, jit=not args.no_jit)
+        res = autotune(ex, rows, jit=not args.no_jit, log=progress)
         tuning = {"best": list(res["best"]), "fwd_ms": {f"{k}/{r}": v for (k, r), v in res["fwd"].items()},
                   "bwd_ms": {f"{k}/{r}": v for (k, r), v in res["bwd"].items()}}
 
@@ -354,6 +386,7 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    progress(f"warmup done ({args.warmup} steps); timing {args.steps} steps")
     probe = EventProbe()
     ex.probe = probe
     torch.cuda.synchronize()
@@ -365,6 +398,7 @@ def main():
     ex.probe = None
     assert torch.isfinite(g).all()
     value = B * args.steps / elapsed
+    progress(f"timed: {value:.1f} evals/s, {elapsed / args.steps * 1e3:.2f} ms per step")
 
     amps = B * (2 ** n)
     kernels = {}
@@ -400,11 +434,14 @@ def main():
     if not args.skip_tn:
         line["tensor_network"] = tn_leg(c4=not args.skip_c4)
     if not args.skip_plain:
-        ps, cnt = time_plain_gpu(n, L, theta[:16], dev)
+        ps, cnt, fwd_s, bwd_s = time_plain_gpu(n, L, theta[:16], dev)
+        progress(f"plain single-circuit path: {ps:.4f} evals/s")
         line["plain_gpu"] = {"value": ps, "unit": "evals/s", "sample": f"{cnt} single-row calls",
+                             "forward_s": fwd_s, "backward_s": bwd_s,
                              "speedup_batched_over_plain": value / ps}
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(n, L, theta[:32])
+        progress(f"cpu baseline: {line['cpu_baseline']['value']:.3f} evals/s")
     print(json.dumps(line), flush=True)
 
 

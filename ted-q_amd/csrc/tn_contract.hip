@@ -15,7 +15,7 @@
 // the address bit of every role bit (role bit j of h, m, n or k -> bit
 // position in A, B or C), so any operand and result layout is allowed.
 //
-// Two kernels:
+// Kernels:
 //   * k_tn_small: output elements in C's address order (coalesced writes), a
 //     group of LPO lanes per element strides over k (lanes adjacent in k, so
 //     an inner product over A's low bits reads contiguous memory), reduced with
@@ -26,7 +26,9 @@
 //     outputs per thread with table-driven operand offsets;
 //   * k_tn_gemm: 64x64 output tiles, k in chunks of 16 staged through LDS,
 //     complex products on the FP32 matrix cores (four real 16x16x4 MFMAs per
-//     complex block and k-step); used when m, n and k are all wide.
+//     complex block and k-step); used when m, n and k are all wide;
+//   * k_tn_gemm128: the same on 128x128 tiles with register prefetch of the
+//     next k chunk, for steps whose m and n are both at least 128 wide.
 // Arithmetic is complex64 with fp32 accumulation, like the reference's
 // complex64 torch contraction.
 #include <stdlib.h>
@@ -244,6 +246,151 @@ __global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restri
   C[b * c_bs + o] = acc;
 }
 
+// Environment-sized results (at most 16 products per summed term, at least
+// 2^12 summed terms): the gradient contractions of gate tensors, where a
+// large environment and a large intermediate are summed over everything but
+// the gate's few indices.  Every thread reads its k elements of A (for all
+// 2^LA values of its (h, m) bits) and of B (all 2^LB values of (h, n)) once
+// and accumulates every output product in registers, so each operand byte is
+// read once (a gather per output element would re-read A 2^nN times and B
+// 2^nM times).  Workgroup s owns summed terms s * 2^(nK - split_log) .. +;
+// thread t takes k = (s << kw) | (it << 8) | t, so the k offsets come from a
+// per-thread part (bits 0..7), an LDS table over the iteration (bits 8..kw-1)
+// and a per-workgroup part.  With one workgroup per batch element
+// (split_log 0) the sums go straight to C; otherwise to part[] in C's address
+// order, and k_tn_env_sum adds the workgroups in a fixed order.
+template <int LA, int LB>
+__global__ void __launch_bounds__(256) k_tn_env(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                float2* __restrict__ C, float2* __restrict__ part,
+                                                const int32_t* __restrict__ dd, int64_t a_bs, int64_t b_bs,
+                                                int64_t c_bs, int split_log, int64_t n_out) {
+  constexpr int NA = 1 << LA, NB = 1 << LB;
+  __shared__ TNShared d;
+  __shared__ uint32_t titA[256], titB[256];
+  __shared__ uint32_t offA[NA], offB[NB], offC[NA * NB];
+  __shared__ float red[4][2 * NA * NB];
+  load_desc(d, dd);
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  const int t = threadIdx.x;
+  const int kw = nK - split_log;      // bits of k one workgroup covers (>= 12)
+  const int iters = 1 << (kw - 8);    // <= 256
+  const uint32_t hmask = (1u << nH) - 1;
+  if (t < iters) {
+    uint32_t oa = 0, ob = 0;
+    for (int j = 8; j < kw; ++j) {
+      const uint32_t bit = ((uint32_t)t >> (j - 8)) & 1u;
+      oa |= bit << d.pos[AK][j];
+      ob |= bit << d.pos[BK][j];
+    }
+    titA[t] = oa;
+    titB[t] = ob;
+  }
+  if (t < NA) offA[t] = scat(d, AH, nH, t & hmask) | scat(d, AM, nM, t >> nH);
+  if (t >= 64 && t < 64 + NB) offB[t - 64] = scat(d, BH, nH, (t - 64) & hmask) | scat(d, BN, nN, (t - 64) >> nH);
+  if (t >= 128 && t < 128 + NA * NB) {
+    const uint32_t i = (t - 128) / NB, j = (t - 128) % NB;
+    offC[t - 128] = scat(d, CH, nH, i & hmask) | scat(d, CM, nM, i >> nH) | scat(d, CN, nN, j >> nH);
+  }
+  __syncthreads();
+  const uint32_t s = blockIdx.x;
+  uint32_t kA = 0, kB = 0;
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t bit = ((uint32_t)t >> j) & 1u;
+    kA |= bit << d.pos[AK][j];
+    kB |= bit << d.pos[BK][j];
+  }
+  for (int j = kw; j < nK; ++j) {
+    const uint32_t bit = (s >> (j - kw)) & 1u;
+    kA |= bit << d.pos[AK][j];
+    kB |= bit << d.pos[BK][j];
+  }
+  const int64_t b = blockIdx.y;
+  const float2* a = A + b * a_bs;
+  const float2* bp = B + b * b_bs;
+  uint32_t oa[NA], ob[NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) oa[i] = offA[i] | kA;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) ob[j] = offB[j] | kB;
+  float2 acc[NA][NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = make_float2(0.f, 0.f);
+  for (int it = 0; it < iters; ++it) {
+    const uint32_t ta = titA[it], tb = titB[it];
+    float2 x[NA], y[NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) x[i] = a[oa[i] | ta];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) y[j] = bp[ob[j] | tb];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        acc[i][j].x = fmaf(x[i].x, y[j].x, fmaf(-x[i].y, y[j].y, acc[i][j].x));
+        acc[i][j].y = fmaf(x[i].x, y[j].y, fmaf(x[i].y, y[j].x, acc[i][j].y));
+      }
+  }
+  // wave sums, then the four waves in LDS
+  const int lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      float re = acc[i][j].x, im = acc[i][j].y;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        re += __shfl_xor(re, o, 64);
+        im += __shfl_xor(im, o, 64);
+      }
+      if (lane == 0) {
+        red[w][2 * (i * NB + j)] = re;
+        red[w][2 * (i * NB + j) + 1] = im;
+      }
+    }
+  __syncthreads();
+  if (t < NA * NB) {
+    const uint32_t i = t / NB, j = t % NB;
+    if ((i & hmask) == (j & hmask)) {
+      float2 v;
+      v.x = (red[0][2 * t] + red[1][2 * t]) + (red[2][2 * t] + red[3][2 * t]);
+      v.y = (red[0][2 * t + 1] + red[1][2 * t + 1]) + (red[2][2 * t + 1] + red[3][2 * t + 1]);
+      if (split_log == 0)
+        C[b * c_bs + offC[t]] = v;
+      else
+        part[((b << split_log) + s) * n_out + offC[t]] = v;
+    }
+  }
+}
+
+// Sum of the k_tn_env workgroup partials: one workgroup per (output element,
+// batch element), 256 threads over the 2^split_log partials, then a fixed
+// tree in LDS (deterministic; a thread per output would be a serial chain).
+__global__ void __launch_bounds__(256) k_tn_env_sum(const float2* __restrict__ part, float2* __restrict__ C,
+                                                    int64_t c_bs, int split_log, int64_t n_out) {
+  __shared__ float2 red[256];
+  const int t = threadIdx.x;
+  const int64_t o = blockIdx.x, b = blockIdx.y;
+  const int S = 1 << split_log;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int s = t; s < S; s += 256) {
+    const float2 v = part[(b * S + s) * n_out + o];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  red[t] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      red[t].x += red[t + h].x;
+      red[t].y += red[t + h].y;
+    }
+    __syncthreads();
+  }
+  if (t == 0) C[b * c_bs + o] = red[0];
+}
+
 // 64x64 output tile of one (batch, h); k staged 16 at a time through LDS; MFMA.
 constexpr int kTile = 64;
 constexpr int kChunk = 16;
@@ -384,12 +531,129 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
       }
 }
 
+// 128 x 128 output tiles for steps with m and n at least 128 wide (whole
+// tiles, no bounds checks): twice the operand reuse of the 64 x 64 form (half
+// the bytes read per multiply-add) and 256 MFMAs per wave per k chunk, with
+// the next chunk's operands fetched into registers while the current one is
+// multiplied.  Wave w owns rows 64 (w & 1) .. +64 and columns 64 (w >> 1) ..
+// +64 as 4 x 4 blocks of 16 x 16.  Each thread loads one k column (t % 16) of
+// eight rows (t / 16 + 16 r) of both operands per chunk.  KID: the summed
+// indices are the lowest address bits of both operands in ascending order (the
+// layout the host gives GEMM operands), so a chunk's k offsets need no
+// scatter.  Requires nM, nN >= 7 and nK >= 4.
+constexpr int kBig = 128;
+constexpr int kBigPad = kBig + 16;  // row stride of 288 words: the four k rows one MFMA read touches use both bank halves
+
+template <bool KID>
+__global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                    float2* __restrict__ C, const int32_t* __restrict__ dd,
+                                                    int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles) {
+  __shared__ float2 As[kChunk][kBigPad];
+  __shared__ float2 Bs[kChunk][kBigPad];
+  __shared__ uint32_t rowC[kBig], colC[kBig];
+  __shared__ TNShared d;
+  load_desc(d, dd);
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  const int t = threadIdx.x;
+  const int64_t b = blockIdx.z;
+  const uint32_t h = blockIdx.y;
+  const int mt = blockIdx.x % m_tiles, nt = blockIdx.x / m_tiles;
+  const uint32_t m0 = (uint32_t)mt * kBig, n0 = (uint32_t)nt * kBig;
+  const uint32_t ksz = 1u << nK;
+  if (t < kBig)
+    rowC[t] = scat(d, CH, nH, h) | scat(d, CM, nM, m0 + t);
+  else
+    colC[t - kBig] = scat(d, CN, nN, n0 + (t - kBig));
+  const float2* a = A + b * a_bs;
+  const float2* bp = B + b * b_bs;
+  const int kk = t & (kChunk - 1), r0 = t >> 4;
+  uint32_t ra[8], rb[8];
+  const uint32_t ha = scat(d, AH, nH, h), hb = scat(d, BH, nH, h);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    ra[r] = ha | scat(d, AM, nM, m0 + r0 + 16 * r);
+    rb[r] = hb | scat(d, BN, nN, n0 + r0 + 16 * r);
+  }
+  float2 pa[8], pb[8];
+  auto fetch = [&](uint32_t k0) {
+    const uint32_t ka = KID ? k0 + kk : scat(d, AK, nK, k0 + kk);
+    const uint32_t kb = KID ? k0 + kk : scat(d, BK, nK, k0 + kk);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      pa[r] = a[ra[r] | ka];
+      pb[r] = bp[rb[r] | kb];
+    }
+  };
+  const int lane = t & 63, w = t >> 6;
+  const int wm = (w & 1) * 64, wn = (w >> 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  f32x4 cr[4][4], ci[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      cr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      ci[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  fetch(0);
+  for (uint32_t k0 = 0; k0 < ksz; k0 += kChunk) {
+    __syncthreads();  // every wave has finished reading the previous chunk
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      As[kk][r0 + 16 * r] = pa[r];
+      Bs[kk][r0 + 16 * r] = pb[r];
+    }
+    __syncthreads();
+    // unconditional (the last chunk re-reads chunk 0, unused), so that the
+    // loads stay in the same block as, and ahead of, the MFMAs below
+    fetch(k0 + kChunk < ksz ? k0 + kChunk : 0u);
+    asm volatile("" ::: "memory");  // issue the loads before the MFMAs, not after them
+#pragma unroll
+    for (int ks = 0; ks < kChunk; ks += 4) {
+      float2 x[4], y[4];
+      float nx[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        x[i] = As[ks + lk][wm + 16 * i + lr];
+        nx[i] = -x[i].y;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) y[j] = Bs[ks + lk][wn + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].x, cr[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].x, y[j].y, ci[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nx[i], y[j].y, cr[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].y, y[j].x, ci[i][j], 0, 0, 0);
+    }
+  }
+  float2* c = C + b * c_bs;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        c[rowC[wm + 16 * i + 4 * lk + v] | colC[wn + 16 * j + lr]] = make_float2(cr[i][j][v], ci[i][j][v]);
+}
+
 // ---------------------------------------------------------------------------
 // launch planning shared by the workspace query and the launch
 // ---------------------------------------------------------------------------
 struct TNLaunch {
   bool gemm;
   bool stream;
+  bool env;
   int lpo_log, split_log;
   int64_t n_out;
 };
@@ -433,7 +697,18 @@ static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
   L.stream = !L.gemm && nK <= 4 && nH + nM + nN >= 12 && (nH + nK + nN <= 8 || nH + nM + nK <= 8);
   L.lpo_log = 0;
   L.split_log = 0;
-  if (L.gemm || L.stream) return L;
+  if (L.gemm || L.stream) {
+    L.env = false;
+    return L;
+  }
+  // a gate-sized result summed over a large rest (k_tn_env): workgroups of 2^12 .. 2^16 summed terms,
+  // at most 2^11 of them
+  L.env = (nH + nM) + (nH + nN) <= 4 && nK >= 12;
+  if (L.env) {
+    const int lo = nK - 16, hi = nK - 12 < 11 ? nK - 12 : 11;
+    L.split_log = lo > hi ? lo : hi;
+    return L;
+  }
   // aim for >= 2^19 threads in flight and <= 2^12 k iterations per thread
   const int64_t want = 1ll << 19;
   int64_t threads = L.n_out * batch;
@@ -483,22 +758,33 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
   const float2* Bp = static_cast<const float2*>(b);
   float2* Cp = static_cast<float2*>(c);
   if (L.gemm) {
-    const int m_tiles = (int)(((1ll << d.nb[1]) + kTile - 1) / kTile);
-    const int64_t n_tiles = ((1ll << d.nb[2]) + kTile - 1) / kTile;
+    // complex products as 4 real MFMA products by default; TEDQ_TN_GEMM=3m selects the 3-product
+    // (Gauss) form, which issues 25% fewer MFMAs but needs a third accumulator set (132 VGPRs
+    // against 90) and measured slower on the 40-qubit chain (191 ms against 170 ms per batch);
+    // TEDQ_TN_GEMM=64 keeps every step on 64 x 64 tiles
+    static const int form = [] {
+      const char* e = getenv("TEDQ_TN_GEMM");
+      if (e == nullptr) return 0;
+      return e[0] == '3' ? 3 : (e[0] == '6' ? 64 : 0);
+    }();
+    const int tile = (form == 0 && d.nb[1] >= 7 && d.nb[2] >= 7) ? kBig : kTile;
+    const int m_tiles = (int)(((1ll << d.nb[1]) + tile - 1) / tile);
+    const int64_t n_tiles = ((1ll << d.nb[2]) + tile - 1) / tile;
     const int64_t gx = m_tiles * n_tiles;
     const int64_t gy = 1ll << d.nb[0];
     if (gx > 0x7fffffffll || gy > 65535) {
       set_error("tedq_tn_contract: grid too large (%lld x %lld)", (long long)gx, (long long)gy);
       return kBadArgument;
     }
-    // complex products as 4 real MFMA products by default; TEDQ_TN_GEMM=3m selects the 3-product
-    // (Gauss) form, which issues 25% fewer MFMAs but needs a third accumulator set (132 VGPRs
-    // against 90) and measured slower on the 40-qubit chain (191 ms against 170 ms per batch)
-    static const bool four_m = [] {
-      const char* e = getenv("TEDQ_TN_GEMM");
-      return e == nullptr || e[0] != '3';
-    }();
-    if (four_m)
+    bool kid = true;
+    for (int j = 0; j < d.nb[3]; ++j) kid = kid && d.pos[AK][j] == j && d.pos[BK][j] == j;
+    if (tile == kBig && kid)
+      hipLaunchKernelGGL(k_tn_gemm128<true>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A,
+                         Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
+    else if (tile == kBig)
+      hipLaunchKernelGGL(k_tn_gemm128<false>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A,
+                         Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
+    else if (form != 3)
       hipLaunchKernelGGL(k_tn_gemm<false>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp,
                          Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
     else
@@ -522,13 +808,33 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
               (long long)workspace_bytes);
     return kBadArgument;
   }
+  float2* part = static_cast<float2*>(workspace);
+  if (L.env) {
+    const int la = d.nb[0] + d.nb[1], lb = d.nb[0] + d.nb[2];
+    const dim3 grid(1u << L.split_log, (unsigned)batch);
+#define TEDQ_ENV_CASE(X, Y)                                                                                       \
+  else if (la == X && lb == Y) hipLaunchKernelGGL((k_tn_env<X, Y>), grid, dim3(256), 0, st, A, Bp, Cp, part,      \
+                                                  desc_dev, a_bstride, b_bstride, c_bstride, L.split_log, L.n_out);
+    if (false) {
+    }
+    TEDQ_ENV_CASE(0, 0) TEDQ_ENV_CASE(0, 1) TEDQ_ENV_CASE(0, 2) TEDQ_ENV_CASE(0, 3) TEDQ_ENV_CASE(0, 4)
+    TEDQ_ENV_CASE(1, 0) TEDQ_ENV_CASE(1, 1) TEDQ_ENV_CASE(1, 2) TEDQ_ENV_CASE(1, 3)
+    TEDQ_ENV_CASE(2, 0) TEDQ_ENV_CASE(2, 1) TEDQ_ENV_CASE(2, 2)
+    TEDQ_ENV_CASE(3, 0) TEDQ_ENV_CASE(3, 1)
+    TEDQ_ENV_CASE(4, 0)
+#undef TEDQ_ENV_CASE
+    int rc = check_launch("tedq_tn_contract(env)");
+    if (rc != kOk || L.split_log == 0) return rc;
+    hipLaunchKernelGGL(k_tn_env_sum, dim3((unsigned)L.n_out, (unsigned)batch), dim3(256), 0, st, part, Cp, c_bstride,
+                       L.split_log, L.n_out);
+    return check_launch("tedq_tn_contract(env sum)");
+  }
   const int64_t threads = L.n_out << (L.lpo_log + L.split_log);
   const int64_t blocks = (threads + 255) / 256;
   if (blocks > 0x7fffffffll) {
     set_error("tedq_tn_contract: grid too large (%lld blocks)", (long long)blocks);
     return kBadArgument;
   }
-  float2* part = static_cast<float2*>(workspace);
   hipLaunchKernelGGL(k_tn_small, dim3((unsigned)blocks, (unsigned)batch), dim3(256), 0, st, A, Bp, Cp, part, desc_dev,
                      a_bstride, b_bstride, c_bstride, L.lpo_log, L.split_log, L.n_out);
   int rc = check_launch("tedq_tn_contract(small)");

@@ -32,12 +32,13 @@ def _time(fn, reps):
     return t0.elapsed_time(t1) / reps
 
 
-def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2, jit=False):
+def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2, jit=False, log=None):
     """Configure ``ex`` (an SVExecutor) with the fastest shapes for ``rows`` ([B, P] on the GPU).
 
     With ``jit`` every candidate runs its generated per-pass kernels (compiled
     once per shape and kept by the executor); shapes those cannot serve are
-    timed with the interpreted kernels.
+    timed with the interpreted kernels.  ``log``, if given, is called with a
+    line of text after every timed candidate.
     Returns {"fwd": {shape: ms}, "bwd": {shape: ms}, "best": (k, r, k_bwd, r_bwd)}.
     """
     n = ex.n
@@ -53,6 +54,8 @@ def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2, jit
         if jit:
             ex.fwd.enable_jit(dev, True, False)
         fwd_ms[(k, r)] = _time(lambda: ex.run_forward(rows), reps)
+        if log:
+            log(f"autotune forward {k}/{r}: {fwd_ms[(k, r)]:.2f} ms")
     kf, rf = min(fwd_ms, key=fwd_ms.get)
     ex.configure(kf, rf, kb0, rb0)
     psi = ex.run_forward(rows)
@@ -63,6 +66,8 @@ def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2, jit
         if jit:
             ex.bwd.enable_jit(dev, False, True)
         bwd_ms[(k, r)] = _time(lambda: ex.run_backward(rows, psi.clone(), lam.clone()), reps)
+        if log:
+            log(f"autotune adjoint {k}/{r}: {bwd_ms[(k, r)]:.2f} ms")
     kb, rb = min(bwd_ms, key=bwd_ms.get)
     ex.configure(kf, rf, kb, rb)
     ex.jit = keep_jit

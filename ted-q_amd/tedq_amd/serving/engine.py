@@ -170,13 +170,18 @@ class ServingEngine:
             return gk
         return None
 
-    def prepare(self, keys, buckets=(1,)):
-        """Compile and capture ahead of traffic (keys above the graph threshold only compile)."""
-        for key in keys:
+    def prepare(self, keys, buckets=(1,), log=None):
+        """Compile and capture ahead of traffic (keys above the graph threshold only compile).
+
+        ``log``, if given, is called with a line of text after every 25 keys."""
+        keys = list(keys)
+        for i, key in enumerate(keys):
             ex = self.executor(key)
             for b in buckets:
                 if b <= self.max_batch:
                     self._graph_for(key, ex, b)
+            if log and ((i + 1) % 25 == 0 or i + 1 == len(keys)):
+                log(f"prepared {i + 1}/{len(keys)} circuit structures")
         torch.cuda.synchronize(self.device)
 
     def _launch_graph(self, key, ex, now):

@@ -53,6 +53,16 @@ CASES = [
     ("ab", "cd", "acbd"),
     ("qabcdefgh", "hgfedcbaz", "zq"),
     ("abcdefghijklmnopqr", "rqponmlkjihgfedcba", ""),
+    # GEMM-sized with n only 64 wide (64 x 64 tiles), with a kept-shared index (128 x 128 tiles),
+    # and with summed indices scattered through both operands' addresses
+    ("abcdefgh" + "ijklmn", "ijklmn" + "opqrst", "abcdefghopqrst"),
+    ("x" + "abcdefg" + "ijkl", "x" + "ijkl" + "mnopqrs", "mnopqrsxabcdefg"),
+    ("iabcjdefghkl", "mnopiqrjklst", "abcdefghmnopqrst"),
+    # gate-environment shapes (few outputs, long sums): plain, with a kept-shared index, and with the
+    # summed indices in opposite orders and the kept ones in the middle of the operands
+    ("ab" + "cdefghijklmnop", "qr" + "cdefghijklmnop", "abqr"),
+    ("x" + "a" + "bcdefghijklm", "x" + "n" + "bcdefghijklm", "nxa"),
+    ("cdaefbghijklmnopqrst", "tsrqponmlkjihgfedc" + "xy", "abxy"),
 ]
 
 

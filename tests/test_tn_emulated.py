@@ -50,6 +50,72 @@ def test_paths_cover_every_tensor_once():
         assert sets[-1] == frozenset({"f"})
 
 
+def test_absorbing_single_qubit_tensors_first():
+    """absorb=4: every rank<=2 tensor is merged into a neighbour before the greedy rule runs, as long
+    as the merged rank stays <= 4 (here every one can be), so no single-qubit tensor meets a larger
+    intermediate later; both paths stay valid and end in the output's index set."""
+    # a 6-qubit brick of 2-qubit gates (rank 4), each qubit line also carrying 1-qubit gates (rank 2)
+    inputs, line, fresh = [], {q: f"q{q}_0" for q in range(6)}, [0]
+
+    def idx():
+        fresh[0] += 1
+        return f"x{fresh[0]}"
+    for q in range(6):
+        inputs.append([line[q]])                                    # |0> (rank 1)
+    for layer in range(4):
+        for q in range(6):
+            o = idx()
+            inputs.append([line[q], o])                             # 1-qubit gate
+            line[q] = o
+        for q in range(layer % 2, 5, 2):
+            o1, o2 = idx(), idx()
+            inputs.append([line[q], line[q + 1], o1, o2])           # 2-qubit gate
+            line[q], line[q + 1] = o1, o2
+    output = [line[q] for q in range(6)]
+    p0, s0 = greedy_path(inputs, output)
+    p1, s1 = greedy_path(inputs, output, absorb=4)
+    n = len(inputs)
+    for p, s in ((p0, s0), (p1, s1)):
+        used = [x for st in p for x in st]
+        assert sorted(used) == list(range(n + len(p) - 1))
+        assert s[-1] == frozenset(output)
+    touches_small = [min(len(s1[i]), len(s1[j])) <= 2 for i, j in p1]
+    prefix = touches_small.index(False)
+    assert prefix > 0 and not any(touches_small[prefix:])          # small tensors only in the prefix
+    assert all(len(s1[n + k]) <= 4 for k in range(prefix))         # ... whose results stay gate-sized
+
+
+def test_lazy_absorption_keeps_single_qubit_tensors_off_large_intermediates():
+    """absorb=-4: a rank<=2 tensor that the greedy rule would contract into a rank>=8 intermediate is
+    merged into a gate-sized neighbour instead (here one always exists); the plain rule does apply
+    single-qubit tensors to the large state, so the two paths differ where it matters."""
+    n_q = 10
+    line = {q: f"s{q}" for q in range(n_q)}
+    inputs = [[line[q] for q in range(n_q)]]                         # a 10-qubit state (rank 10)
+    fresh = [0]
+
+    def idx():
+        fresh[0] += 1
+        return f"x{fresh[0]}"
+    for layer in range(3):
+        for q in range(n_q):
+            o = idx()
+            inputs.append([line[q], o])
+            line[q] = o
+        for q in range(layer % 2, n_q - 1, 2):
+            o1, o2 = idx(), idx()
+            inputs.append([line[q], line[q + 1], o1, o2])
+            line[q], line[q + 1] = o1, o2
+    output = [line[q] for q in range(n_q)]
+    n = len(inputs)
+    for absorb, expect_small_on_large in ((0, True), (-4, False)):
+        p, s = greedy_path(inputs, output, 4.0, 0.0, 0, absorb)
+        assert sorted(x for st in p for x in st) == list(range(n + len(p) - 1))
+        assert s[-1] == frozenset(output)
+        small_on_large = any(min(len(s[i]), len(s[j])) <= 2 and max(len(s[i]), len(s[j])) >= 8 for i, j in p)
+        assert small_on_large == expect_small_on_large, absorb
+
+
 def _random_circuit(rng, n, depth):
     """A random circuit as (frontend builder, oracle gate list as a function of the parameter vector)."""
     spec = []
