@@ -251,10 +251,19 @@ def measured_traffic(kernel, n, layers, batch, ex):
     return rec["traffic_bytes_per_launch"] if same else None
 
 
-def tn_leg(c4=True):
+def tn_leg(c4=True, c3=True):
     """Contraction mode: the 40-qubit disordered chain (forward, batch 32) and the 5x5 lattice (fwd + grad)."""
     from tedq_amd import workloads as W  # pylint: disable=import-outside-toplevel
     dev = torch.device("cuda", 0)
+    out = {}
+    if c3:
+        out.update(_chain_leg(W, dev))
+    if c4:
+        out.update(_lattice_leg(W, dev))
+    return out
+
+
+def _chain_leg(W, dev):
     out = {}
     t0 = time.perf_counter()
     ex = W.disordered_chain(40, 8, 20, tn_mode=True).executor()
@@ -264,7 +273,7 @@ def tn_leg(c4=True):
     with torch.no_grad():
         ex.measure(ex.run_forward(rows))
         torch.cuda.synchronize()
-        reps = 3
+        reps = 10
         t0 = time.perf_counter()
         for _ in range(reps):
             v = ex.measure(ex.run_forward(rows))[0]
@@ -277,28 +286,32 @@ def tn_leg(c4=True):
                       "steps": info["steps"], "largest_rank": info["largest_rank"], "macs_per_eval": info["macs"],
                       "achieved_TFLOPs": 8.0 * info["macs"] * 32 / dt / 1e12,
                       "mean_value": float(v.mean())}
-    if c4:
-        t0 = time.perf_counter()
-        ex = W.lattice(5, 6, tn_mode=True).executor()
-        build_s = time.perf_counter() - t0
-        progress(f"contraction: 5x5 lattice paths found in {build_s:.1f} s")
-        rng = np.random.default_rng(3)
-        rows = torch.tensor(rng.uniform(0, 2 * np.pi, size=(1, 300)), dtype=torch.float32, device=dev)
-        ups = [torch.full((1,), 1.0 / 25, device=dev) for _ in range(25)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        outs, grad = ex.value_and_grad(rows, ups)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        rep = ex.path_report()
-        progress(f"contraction: 5x5 lattice value and gradient in {dt:.1f} s")
-        out["lattice5x5"] = {"workload": "5x5 lattice 6 layers, mean <Z_i> over 25 networks, fwd + reverse-tree grad",
-                             "s_per_eval": dt, "path_build_s": build_s,
-                             "steps": sum(r["steps"] for r in rep),
-                             "largest_rank": max(r["largest_rank"] for r in rep),
-                             "macs_fwd": sum(r["macs"] for r in rep),
-                             "cost": float(sum(float(o[0]) for o in outs) / 25),
-                             "grad_finite": bool(torch.isfinite(grad).all())}
+    return out
+
+
+def _lattice_leg(W, dev):
+    out = {}
+    t0 = time.perf_counter()
+    ex = W.lattice(5, 6, tn_mode=True).executor()
+    build_s = time.perf_counter() - t0
+    progress(f"contraction: 5x5 lattice paths found in {build_s:.1f} s")
+    rng = np.random.default_rng(3)
+    rows = torch.tensor(rng.uniform(0, 2 * np.pi, size=(1, 300)), dtype=torch.float32, device=dev)
+    ups = [torch.full((1,), 1.0 / 25, device=dev) for _ in range(25)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs, grad = ex.value_and_grad(rows, ups)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rep = ex.path_report()
+    progress(f"contraction: 5x5 lattice value and gradient in {dt:.1f} s")
+    out["lattice5x5"] = {"workload": "5x5 lattice 6 layers, mean <Z_i> over 25 networks, fwd + reverse-tree grad",
+                         "s_per_eval": dt, "path_build_s": build_s,
+                         "steps": sum(r["steps"] for r in rep),
+                         "largest_rank": max(r["largest_rank"] for r in rep),
+                         "macs_fwd": sum(r["macs"] for r in rep),
+                         "cost": float(sum(float(o[0]) for o in outs) / 25),
+                         "grad_finite": bool(torch.isfinite(grad).all())}
     return out
 
 
@@ -320,7 +333,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--skip-tn", action="store_true")
     ap.add_argument("--no-jit", action="store_true", help="keep the interpreted pass kernels")
-    ap.add_argument("--skip-c4", action="store_true")
+    ap.add_argument("--skip-c4", action="store_true", help="leave out the 5x5 lattice (contraction mode)")
+    ap.add_argument("--skip-c3", action="store_true", help="leave out the 40-qubit chain (contraction mode)")
     ap.add_argument("--tn-only", action="store_true", help="run only the contraction-mode leg and print its JSON")
     ap.add_argument("--plain-only", action="store_true",
                     help="time one call of the plain per-gate torch path (--qubits, --layers) and print its JSON")
@@ -343,7 +357,7 @@ def main():
 
     import tedq_amd  # noqa: F401  pylint: disable=import-outside-toplevel,unused-import
     if args.tn_only:
-        print(json.dumps({"tensor_network": tn_leg(c4=not args.skip_c4)}), flush=True)
+        print(json.dumps({"tensor_network": tn_leg(c4=not args.skip_c4, c3=not args.skip_c3)}), flush=True)
         return
     if args.plain_only:
         theta = np.random.default_rng(1).uniform(0, 2 * np.pi, size=(2, args.layers * args.qubits * 2))

@@ -24,6 +24,9 @@
 //     k_tn_split_sum (deterministic, no atomics).
 //   * k_tn_stream: the skinny steps (a large tensor times a gate), 16
 //     outputs per thread with table-driven operand offsets;
+//   * k_tn_gate: the common case of those (at most two summed and two new
+//     indices, nothing kept-shared): each large-operand row read once;
+//   * k_tn_env: few outputs summed over a long range (gate environments);
 //   * k_tn_gemm: 64x64 output tiles, k in chunks of 16 staged through LDS,
 //     complex products on the FP32 matrix cores (four real 16x16x4 MFMAs per
 //     complex block and k-step); used when m, n and k are all wide;
@@ -229,6 +232,83 @@ __global__ void __launch_bounds__(256) k_tn_stream(const float2* __restrict__ A,
     }
     c[j << 8] = acc;
   }
+}
+
+// A large tensor times a gate-sized one without kept-shared indices (the
+// gate applications of state-like intermediates): every thread owns four rows
+// p of the large operand X (its kept indices), reads X[p, k] for all 2^NK
+// summed values once and writes C[p, q] = sum_k G[q, k] X[p, k] for all 2^NQ
+// gate outputs, so X is read once and C written once (the general streaming
+// form reads X once per output).  Rows are numbered in C's address order; row
+// r = (g * 4 + j) * 256 + t takes its offsets from a per-thread part (row bits
+// 0..7), a four-entry LDS table (bits 8..9) and a per-workgroup part.  XBIG:
+// A is the large operand (else B; the product commutes).  Needs nH = 0 and at
+// least 2^10 rows.
+template <int NK, int NQ>
+__global__ void __launch_bounds__(256) k_tn_gate(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                 float2* __restrict__ C, const int32_t* __restrict__ dd,
+                                                 int64_t a_bs, int64_t b_bs, int64_t c_bs, int a_big) {
+  constexpr int KN = 1 << NK, QN = 1 << NQ;
+  __shared__ TNShared d;
+  __shared__ uint32_t midX[4], midC[4];
+  load_desc(d, dd);
+  const int XP = a_big ? AM : BN, XK = a_big ? AK : BK, GQ = a_big ? BN : AM, GK = a_big ? BK : AK;
+  const int CP = a_big ? CM : CN, CQ = a_big ? CN : CM;
+  const int nP = d.nb[a_big ? 1 : 2];
+  const int t = threadIdx.x;
+  if (t < 4) {
+    uint32_t ox = 0, oc = 0;
+    for (int j = 8; j < 10; ++j) {
+      const uint32_t bit = ((uint32_t)t >> (j - 8)) & 1u;
+      ox |= bit << d.pos[XP][j];
+      oc |= bit << d.pos[CP][j];
+    }
+    midX[t] = ox;
+    midC[t] = oc;
+  }
+  __syncthreads();
+  const float2* X = a_big ? A + blockIdx.y * a_bs : B + blockIdx.y * b_bs;
+  const float2* G = a_big ? B + blockIdx.y * b_bs : A + blockIdx.y * a_bs;
+  float2 g[QN][KN];
+  uint32_t xk[KN], cq[QN];
+#pragma unroll
+  for (int k = 0; k < KN; ++k) xk[k] = scat(d, XK, NK, k);
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+    cq[q] = scat(d, CQ, NQ, q);
+#pragma unroll
+    for (int k = 0; k < KN; ++k) g[q][k] = G[scat(d, GQ, NQ, q) | scat(d, GK, NK, k)];
+  }
+  uint32_t loX = 0, loC = 0, hiX = 0, hiC = 0;
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t bit = ((uint32_t)t >> j) & 1u;
+    loX |= bit << d.pos[XP][j];
+    loC |= bit << d.pos[CP][j];
+  }
+  const uint32_t blk = blockIdx.x;
+  for (int j = 10; j < nP; ++j) {
+    const uint32_t bit = (blk >> (j - 10)) & 1u;
+    hiX |= bit << d.pos[XP][j];
+    hiC |= bit << d.pos[CP][j];
+  }
+  float2* c = C + blockIdx.y * c_bs;
+  float2 x[4][KN];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int k = 0; k < KN; ++k) x[j][k] = X[hiX | midX[j] | loX | xk[k]];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < KN; ++k) {
+        acc.x = fmaf(g[q][k].x, x[j][k].x, fmaf(-g[q][k].y, x[j][k].y, acc.x));
+        acc.y = fmaf(g[q][k].x, x[j][k].y, fmaf(g[q][k].y, x[j][k].x, acc.y));
+      }
+      c[hiC | midC[j] | loC | cq[q]] = acc;
+    }
 }
 
 __global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restrict__ C, int64_t c_bs, int split_log,
@@ -535,12 +615,14 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
 // tiles, no bounds checks): twice the operand reuse of the 64 x 64 form (half
 // the bytes read per multiply-add) and 256 MFMAs per wave per k chunk, with
 // the next chunk's operands fetched into registers while the current one is
-// multiplied.  Wave w owns rows 64 (w & 1) .. +64 and columns 64 (w >> 1) ..
-// +64 as 4 x 4 blocks of 16 x 16.  Each thread loads one k column (t % 16) of
-// eight rows (t / 16 + 16 r) of both operands per chunk.  KID: the summed
-// indices are the lowest address bits of both operands in ascending order (the
-// layout the host gives GEMM operands), so a chunk's k offsets need no
-// scatter.  Requires nM, nN >= 7 and nK >= 4.
+// multiplied (double-buffered in LDS).  Wave w owns rows 64 (w & 1) .. +64
+// and columns 64 (w >> 1) .. +64 as 4 x 4 blocks of 16 x 16; measured
+// 107-114 TFLOP/s (0.68-0.72 of the FP32 matrix peak) on 2^10-2^12 cubes.
+// Each thread loads one k column (t % 16) of eight rows (t / 16 + 16 r) of
+// both operands per chunk.  KID: the summed indices are the lowest address
+// bits of both operands in ascending order (the layout the host gives GEMM
+// operands), so a chunk's k offsets need no scatter.  Requires nM, nN >= 7
+// and nK >= 4.
 constexpr int kBig = 128;
 constexpr int kBigPad = kBig + 16;  // row stride of 288 words: the four k rows one MFMA read touches use both bank halves
 
@@ -548,8 +630,8 @@ template <bool KID>
 __global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A, const float2* __restrict__ B,
                                                     float2* __restrict__ C, const int32_t* __restrict__ dd,
                                                     int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles) {
-  __shared__ float2 As[kChunk][kBigPad];
-  __shared__ float2 Bs[kChunk][kBigPad];
+  __shared__ float2 As2[2][kChunk][kBigPad];
+  __shared__ float2 Bs2[2][kChunk][kBigPad];
   __shared__ uint32_t rowC[kBig], colC[kBig];
   __shared__ TNShared d;
   load_desc(d, dd);
@@ -595,19 +677,24 @@ __global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A
       cr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       ci[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
+  // LDS holds two chunks: chunk i is multiplied from buffer i % 2 while chunk
+  // i + 1 (fetched into registers meanwhile) goes into the other buffer, so
+  // one barrier per chunk suffices
   fetch(0);
-  for (uint32_t k0 = 0; k0 < ksz; k0 += kChunk) {
-    __syncthreads();  // every wave has finished reading the previous chunk
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      As[kk][r0 + 16 * r] = pa[r];
-      Bs[kk][r0 + 16 * r] = pb[r];
-    }
-    __syncthreads();
+  for (int r = 0; r < 8; ++r) {
+    As2[0][kk][r0 + 16 * r] = pa[r];
+    Bs2[0][kk][r0 + 16 * r] = pb[r];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (uint32_t k0 = 0; k0 < ksz; k0 += kChunk) {
     // unconditional (the last chunk re-reads chunk 0, unused), so that the
     // loads stay in the same block as, and ahead of, the MFMAs below
     fetch(k0 + kChunk < ksz ? k0 + kChunk : 0u);
     asm volatile("" ::: "memory");  // issue the loads before the MFMAs, not after them
+    float2(*As)[kBigPad] = As2[cur];
+    float2(*Bs)[kBigPad] = Bs2[cur];
 #pragma unroll
     for (int ks = 0; ks < kChunk; ks += 4) {
       float2 x[4], y[4];
@@ -636,6 +723,13 @@ __global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A
 #pragma unroll
         for (int j = 0; j < 4; ++j) ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i].y, y[j].x, ci[i][j], 0, 0, 0);
     }
+    cur ^= 1;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      As2[cur][kk][r0 + 16 * r] = pa[r];
+      Bs2[cur][kk][r0 + 16 * r] = pb[r];
+    }
+    __syncthreads();
   }
   float2* c = C + b * c_bs;
 #pragma unroll
@@ -791,6 +885,22 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
       hipLaunchKernelGGL(k_tn_gemm<true>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp,
                          Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
     return check_launch("tedq_tn_contract(gemm)");
+  }
+  if (L.stream && d.nb[0] == 0 && d.nb[3] <= 2 && ((d.nb[2] <= 2 && d.nb[1] >= 10) || (d.nb[1] <= 2 && d.nb[2] >= 10))) {
+    // a gate of at most two summed and two new indices on a large tensor: k_tn_gate
+    const int a_big = d.nb[1] >= 10 ? 1 : 0;
+    const int nq = a_big ? d.nb[2] : d.nb[1], nk = d.nb[3], np = a_big ? d.nb[1] : d.nb[2];
+    const dim3 grid(1u << (np - 10), (unsigned)batch);
+#define TEDQ_GATE_CASE(K, Q)                                                                                  \
+  else if (nk == K && nq == Q) hipLaunchKernelGGL((k_tn_gate<K, Q>), grid, dim3(256), 0, st, A, Bp, Cp, desc_dev, \
+                                                  a_bstride, b_bstride, c_bstride, a_big);
+    if (false) {
+    }
+    TEDQ_GATE_CASE(0, 0) TEDQ_GATE_CASE(0, 1) TEDQ_GATE_CASE(0, 2)
+    TEDQ_GATE_CASE(1, 0) TEDQ_GATE_CASE(1, 1) TEDQ_GATE_CASE(1, 2)
+    TEDQ_GATE_CASE(2, 0) TEDQ_GATE_CASE(2, 1) TEDQ_GATE_CASE(2, 2)
+#undef TEDQ_GATE_CASE
+    return check_launch("tedq_tn_contract(gate)");
   }
   if (L.stream) {
     const int64_t blocks = L.n_out >> 12;

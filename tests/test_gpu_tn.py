@@ -63,6 +63,12 @@ CASES = [
     ("ab" + "cdefghijklmnop", "qr" + "cdefghijklmnop", "abqr"),
     ("x" + "a" + "bcdefghijklm", "x" + "n" + "bcdefghijklm", "nxa"),
     ("cdaefbghijklmnopqrst", "tsrqponmlkjihgfedc" + "xy", "abxy"),
+    # gates on a large tensor: two-qubit gate (A large), one-qubit gate with the large tensor second,
+    # gate indices in the middle of both layouts, and a large operand with the gate's indices at both ends
+    ("abcdefghijkl" + "xy", "xyuv", "abcdefghijkl" + "uv"),
+    ("xy", "x" + "abcdefghijkl", "y" + "abcdefghijkl"),
+    ("abcdexfghijkl", "xu", "abcudefghijkl"),
+    ("uvxy", "abcdefxghijkly", "uabcdefghijklv"),
 ]
 
 
