@@ -27,6 +27,8 @@
 //   * k_tn_gate: the common case of those (at most two summed and two new
 //     indices, nothing kept-shared): each large-operand row read once;
 //   * k_tn_env: few outputs summed over a long range (gate environments);
+//   * k_tn_permute: a tensor times a scalar into another layout (operand
+//     copies), transposed through LDS tiles;
 //   * k_tn_gemm: 64x64 output tiles, k in chunks of 16 staged through LDS,
 //     complex products on the FP32 matrix cores (four real 16x16x4 MFMAs per
 //     complex block and k-step); used when m, n and k are all wide;
@@ -309,6 +311,115 @@ __global__ void __launch_bounds__(256) k_tn_gate(const float2* __restrict__ A, c
       }
       c[hiC | midC[j] | loC | cq[q]] = acc;
     }
+}
+
+// A tensor times a scalar into another index layout (the copies of GEMM
+// operands into k-contiguous layouts): a permutation of address bits.  Each
+// workgroup moves tiles of 2^10 elements: the tile bits are the index bits at
+// the five lowest source and the five lowest destination address bits (topped
+// up with the next destination bits), so the loads (in source order) and the
+// stores (in destination order) both run along contiguous addresses, and LDS
+// carries the transpose.  The remaining bits number the tiles.  The host
+// derives the bit lists once per launch (TNPermPlan, passed by value).
+constexpr int kPermBits = 10;
+constexpr int kPermTilesPerGroup = 4;
+
+struct TNPermPlan {
+  int n_rest;                    // index bits outside the tile
+  int a_tensor;                  // A is the tensor, B the scalar (else the reverse)
+  unsigned char src_t[kPermBits];  // source address bit of tile bit q, tile bits in source order
+  unsigned char dst_t[kPermBits];  // destination address bit of tile bit q, tile bits in destination order
+  unsigned char s_of_d[kPermBits];  // source-order number of the destination-order tile bit q
+  unsigned char src_r[32], dst_r[32];  // source / destination address bit of rest bit q
+};
+
+__global__ void __launch_bounds__(256) k_tn_permute(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                    float2* __restrict__ C, int64_t a_bs, int64_t b_bs,
+                                                    int64_t c_bs, TNPermPlan P) {
+  constexpr int N = 1 << kPermBits;
+  __shared__ float2 tile[N + N / 32];  // one pad word per 32 elements
+  __shared__ uint32_t tabA[N];         // source offset part of the i-th element in source order
+  __shared__ uint32_t tabC[N];         // destination offset part of the o-th element in destination order
+  __shared__ uint16_t tabI[N];         // source-order number of the o-th element in destination order
+  const int t = threadIdx.x;
+  for (int e = t; e < N; e += 256) {
+    uint32_t sa = 0, oc = 0, is = 0;
+#pragma unroll
+    for (int q = 0; q < kPermBits; ++q) {
+      const uint32_t bs = (e >> q) & 1u;
+      sa |= bs << P.src_t[q];
+      oc |= bs << P.dst_t[q];
+      is |= bs << P.s_of_d[q];
+    }
+    tabA[e] = sa;
+    tabC[e] = oc;
+    tabI[e] = (uint16_t)is;
+  }
+  __syncthreads();
+  const float2* x = P.a_tensor ? A + blockIdx.y * a_bs : B + blockIdx.y * b_bs;
+  const float2 s = P.a_tensor ? B[blockIdx.y * b_bs] : A[blockIdx.y * a_bs];
+  float2* c = C + blockIdx.y * c_bs;
+  for (int u = 0; u < kPermTilesPerGroup; ++u) {
+    const uint32_t g = blockIdx.x * kPermTilesPerGroup + u;
+    uint32_t baseA = 0, baseC = 0;
+    for (int q = 0; q < P.n_rest; ++q) {
+      const uint32_t bit = (g >> q) & 1u;
+      baseA |= bit << P.src_r[q];
+      baseC |= bit << P.dst_r[q];
+    }
+#pragma unroll
+    for (int j = 0; j < N / 256; ++j) {
+      const int i = t + 256 * j;
+      tile[i + (i >> 5)] = x[baseA | tabA[i]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < N / 256; ++j) {
+      const int o = t + 256 * j;
+      const int i = tabI[o];
+      const float2 v = tile[i + (i >> 5)];
+      c[baseC | tabC[o]] = make_float2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
+    }
+    __syncthreads();
+  }
+}
+
+// Tile and rest bits of a permutation step (role bits of the tensor operand:
+// role bit j sits at source bit src[j] and destination bit dst[j]).
+static TNPermPlan perm_plan(const TNDesc& d, int a_tensor) {
+  TNPermPlan P;
+  const int XR = a_tensor ? AM : BN, CR = a_tensor ? CM : CN;
+  const int r = d.nb[a_tensor ? 1 : 2];
+  int role_at_src[32], role_at_dst[32];
+  for (int j = 0; j < r; ++j) {
+    role_at_src[(int)d.pos[XR][j]] = j;
+    role_at_dst[(int)d.pos[CR][j]] = j;
+  }
+  uint32_t in_tile = 0;
+  for (int p = 0; p < 5; ++p) in_tile |= (1u << role_at_src[p]) | (1u << role_at_dst[p]);
+  for (int p = 5; p < r && __builtin_popcount(in_tile) < kPermBits; ++p) in_tile |= 1u << role_at_dst[p];
+  int nt = 0, nr = 0, ns = 0;
+  int s_number[32];
+  for (int p = 0; p < r; ++p) {
+    const int j = role_at_src[p];
+    if ((in_tile >> j) & 1u) {
+      s_number[j] = ns;
+      P.src_t[ns++] = (unsigned char)p;
+    }
+  }
+  for (int p = 0; p < r; ++p) {
+    const int j = role_at_dst[p];
+    if ((in_tile >> j) & 1u) {
+      P.s_of_d[nt] = (unsigned char)s_number[j];
+      P.dst_t[nt++] = (unsigned char)p;
+    } else {
+      P.src_r[nr] = (unsigned char)d.pos[XR][j];
+      P.dst_r[nr++] = (unsigned char)p;
+    }
+  }
+  P.n_rest = nr;
+  P.a_tensor = a_tensor;
+  return P;
 }
 
 __global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restrict__ C, int64_t c_bs, int split_log,
@@ -885,6 +996,16 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
       hipLaunchKernelGGL(k_tn_gemm<true>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp,
                          Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
     return check_launch("tedq_tn_contract(gemm)");
+  }
+  if (d.nb[0] == 0 && d.nb[3] == 0 && (d.nb[1] == 0 || d.nb[2] == 0) && d.nb[1] + d.nb[2] >= kPermBits + 2) {
+    // a tensor times a scalar: a layout change through LDS tiles (k_tn_permute)
+    const int a_tensor = d.nb[1] > 0 ? 1 : 0;
+    const int r = d.nb[1] + d.nb[2];
+    const TNPermPlan P = perm_plan(d, a_tensor);
+    const unsigned groups = (1u << (r - kPermBits)) / kPermTilesPerGroup;
+    hipLaunchKernelGGL(k_tn_permute, dim3(groups, (unsigned)batch), dim3(256), 0, st, A, Bp, Cp, a_bstride,
+                       b_bstride, c_bstride, P);
+    return check_launch("tedq_tn_contract(permute)");
   }
   if (L.stream && d.nb[0] == 0 && d.nb[3] <= 2 && ((d.nb[2] <= 2 && d.nb[1] >= 10) || (d.nb[1] <= 2 && d.nb[2] >= 10))) {
     // a gate of at most two summed and two new indices on a large tensor: k_tn_gate

@@ -69,6 +69,11 @@ CASES = [
     ("xy", "x" + "abcdefghijkl", "y" + "abcdefghijkl"),
     ("abcdexfghijkl", "xu", "abcudefghijkl"),
     ("uvxy", "abcdefxghijkly", "uabcdefghijklv"),
+    # a tensor times a scalar into another layout (tiled transpose): reversed, low and high bits swapped,
+    # and the scalar first
+    ("abcdefghijklmn", "", "nmlkjihgfedcba"),
+    ("abcdefghijklmn", "", "jklmnabcdefghi"),
+    ("", "abcdefghijklm", "dgmbjelahcfik"),
 ]
 
 
