@@ -141,13 +141,30 @@ def time_plain_gpu(n, layers, theta, device, budget_s=8.0):
     return done / (time.perf_counter() - t0), done, fwd_s / done, bwd_s / done
 
 
-def cpu_baseline(n, layers, theta, budget_s=15.0):
-    """The oracle's complex64 restatement on the host cores (bounded sample)."""
+def cpu_baseline(n, layers, theta, budget_s=15.0, check=None):
+    """The oracle's complex64 restatement on the host cores (bounded sample).
+
+    ``check`` = (row indices, GPU values [r, n], GPU gradients [r, P]) of the
+    same workload: those rows are also run through the oracle in complex128 and
+    the relative errors max|x - x128| / max(max|x128|, 1e-12) are returned."""
     from oracle import sv_oracle as O  # pylint: disable=import-outside-toplevel
     # the host threads this process may use: the OMP_NUM_THREADS share when set, else its CPU affinity
     cores = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
     meas = [("expval", ("PauliZ", [q])) for q in range(n)]
+    error = None
+    if check is not None:
+        idx, vals, grads = check
+        ev = eg = 0.0
+        for r, i in enumerate(idx):
+            t = torch.tensor(theta[i].reshape(layers, n, 2), dtype=torch.float64, requires_grad=True)
+            ref = O.execute(n, O.hea_gates(n, t)[0], meas)
+            ref.sum().backward()
+            rv, rg = ref.detach().numpy(), t.grad.reshape(-1).numpy()
+            ev = max(ev, float(np.abs(vals[r] - rv).max() / max(np.abs(rv).max(), 1e-12)))
+            eg = max(eg, float(np.abs(grads[r] - rg).max() / max(np.abs(rg).max(), 1e-12)))
+        error = {"rows": [int(i) for i in idx], "values_rel": ev, "gradients_rel": eg,
+                 "reference": "oracle in complex128 (torch autograd), same parameter rows"}
     done = 0
     t0 = time.perf_counter()
     for i in range(theta.shape[0]):
@@ -164,10 +181,13 @@ def cpu_baseline(n, layers, theta, budget_s=15.0):
                  .stdout.splitlines() if ln.startswith("Model name")][0]
     except (OSError, IndexError):
         pass
-    return {"value": done / el, "unit": "evals/s", "cores": cores, "kind": "port",
-            "sample": f"{done} of {theta.shape[0]} parameter rows, {n}q d{layers} fwd+grad, complex64 "
-                      f"tensordot+permute per gate with torch autograd, {el:.1f} s",
-            "cpu_model": model}
+    out = {"value": done / el, "unit": "evals/s", "cores": cores, "kind": "port",
+           "sample": f"{done} of {theta.shape[0]} parameter rows, {n}q d{layers} fwd+grad, complex64 "
+                     f"tensordot+permute per gate with torch autograd, {el:.1f} s",
+           "cpu_model": model}
+    if error is not None:
+        out["error_vs_complex128"] = error
+    return out
 
 
 def hea_z0_circuit(n, layers):
@@ -442,6 +462,11 @@ def main():
                      "traffic": measured_traffic(dom, n, L, B, ex),
                      "bytes_per_launch": kd["bytes_per_launch"], "mean_launch_ms": kd["mean_ms"]},
         "kernels": kernels,
+        # speed figure: the sweeps an unfused gate-by-gate engine makes per evaluation (3 per gate in the
+        # forward + adjoint, one per parameter, one for the measurement), 16 B per amplitude each;
+        # fusion lets this exceed HBM bandwidth, so it is never a roofline fraction
+        "unfused_equivalent": {"sweeps_per_eval": 3 * L * (3 * n - 1) + 2 * n * L + 1,
+                               "TB_per_s": value * (3 * L * (3 * n - 1) + 2 * n * L + 1) * 16 * 2 ** n / 1e12},
     }
     if not args.skip_serving:
         line["latency"] = serving_leg(args.serve_requests, warm=200)
@@ -454,7 +479,13 @@ def main():
                              "forward_s": fwd_s, "backward_s": bwd_s,
                              "speedup_batched_over_plain": value / ps}
     if not args.skip_cpu:
-        line["cpu_baseline"] = cpu_baseline(n, L, theta[:32])
+        # the engine's values and gradients of two rows of the benchmarked batch, for the error check
+        idx = [0, min(B, 32) - 1]
+        r = rows[idx].detach().clone().requires_grad_(True)
+        vals = compiled.execute_batch(r)
+        vals.sum().backward()
+        check = (idx, vals.detach().double().cpu().numpy(), r.grad.double().cpu().numpy())
+        line["cpu_baseline"] = cpu_baseline(n, L, theta[:32], check=check)
         progress(f"cpu baseline: {line['cpu_baseline']['value']:.3f} evals/s")
     print(json.dumps(line), flush=True)
 

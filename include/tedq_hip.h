@@ -150,6 +150,20 @@ int64_t tedq_tn_workspace_bytes(const int32_t* desc, int64_t batch);
 int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b, int64_t b_bstride, void* c,
                      int64_t c_bstride, int64_t batch, const int32_t* desc, const int32_t* desc_dev,
                      void* workspace, int64_t workspace_bytes, void* stream);
+/* The small steps of one level of a contraction tree in one launch (steps whose
+ * operands are leaves or results of earlier small steps: the gate merges that
+ * make up most of a path).  bases/strides/lens: three operand arenas as device
+ * pointers, per-batch strides in elements (0 = shared) and valid elements per
+ * batch row; every result goes to arena 2.  records (host) / records_dev
+ * (device): n_records x 8 int64 {desc row, A arena, A offset, B arena,
+ * B offset, C offset, output count, 0}; descs / descs_dev: the descriptor table
+ * (n_descs rows of 4 + 9*32 int32, layout as tedq_tn_contract).  Operands and
+ * results of rank <= 12, at most 2^6 summed terms, bounds checked on the host.
+ * Replaces the same pairwise steps of tree.contract(arrays, backend='torch')
+ * (pytorch_backend.py:276-354) as tedq_tn_contract, issued together. */
+int tedq_tn_contract_small_batch(const void* const* bases, const int64_t* strides, const int64_t* lens,
+                                 const int64_t* records, const int64_t* records_dev, int n_records, int64_t batch,
+                                 const int32_t* descs, const int32_t* descs_dev, int n_descs, void* stream);
 /* Gate tensors of the network's leaves for every parameter row: ket U with
  * axes (out..., in...) or bra U^dagger, and dV/dtheta 2x2 blocks of trainable
  * parameters.  recs: dev int32[16] records {code, src0..2, const_off, nq,

@@ -422,6 +422,56 @@ static TNPermPlan perm_plan(const TNDesc& d, int a_tensor) {
   return P;
 }
 
+// Many small steps in one launch (the gate-merging steps of a contraction
+// path: every operand a leaf or the result of another small step).  Workgroup
+// (r, b) runs step r of the list for batch element b: outputs in C's address
+// order over the threads, each summed over all 2^nK terms.  A record is eight
+// int64: descriptor row, A base id, A element offset, B base id, B offset,
+// C offset (in base 2), output count, reserved; a base id selects one of the
+// three (pointer, batch stride) pairs passed by value.
+struct TNBases {
+  const float2* p[3];
+  int64_t stride[3];
+};
+
+__global__ void __launch_bounds__(256) k_tn_small_multi(TNBases bases, float2* __restrict__ out_base,
+                                                        const int64_t* __restrict__ recs,
+                                                        const int32_t* __restrict__ descs) {
+  __shared__ TNShared d;
+  __shared__ signed char mapA[32], mapB[32];
+  __shared__ uint32_t tka[64], tkb[64];
+  const int64_t* rec = recs + 8 * (int64_t)blockIdx.x;
+  const int row = (int)rec[0];
+  load_desc(d, descs + (int64_t)row * (4 + 9 * 32));
+  c_bit_maps(d, mapA, mapB);
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  const int rc = nH + nM + nN;
+  const int t = threadIdx.x;
+  if (t < (1 << nK)) {
+    tka[t] = scat(d, AK, nK, t);
+    tkb[t] = scat(d, BK, nK, t);
+  }
+  __syncthreads();
+  const int64_t b = blockIdx.y;
+  const int ia = (int)rec[1], ib = (int)rec[3];
+  const float2* a = bases.p[ia] + b * bases.stride[ia] + rec[2];
+  const float2* bp = bases.p[ib] + b * bases.stride[ib] + rec[4];
+  float2* c = out_base + b * bases.stride[2] + rec[5];
+  const int n_out = (int)rec[6];
+  const int kc = 1 << nK;
+  for (int o = t; o < n_out; o += 256) {
+    const uint32_t oa = via_map(mapA, rc, (uint32_t)o), ob = via_map(mapB, rc, (uint32_t)o);
+    float2 acc = make_float2(0.f, 0.f);
+    for (int k = 0; k < kc; ++k) {
+      const float2 x = a[oa | tka[k]];
+      const float2 y = bp[ob | tkb[k]];
+      acc.x = fmaf(x.x, y.x, fmaf(-x.y, y.y, acc.x));
+      acc.y = fmaf(x.x, y.y, fmaf(x.y, y.x, acc.y));
+    }
+    c[o] = acc;
+  }
+}
+
 __global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restrict__ C, int64_t c_bs, int split_log,
                                int64_t n_out) {
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1075,3 +1125,53 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
                      L.split_log, L.n_out);
   return check_launch("tedq_tn_contract(split sum)");
 }
+
+// Small steps of one tree level in one launch (see k_tn_small_multi).  bases /
+// strides / lens: three operand arenas (device pointers, per-batch strides in
+// elements, elements valid per batch row); base 2 is also where every result
+// goes.  records / records_dev: n_records records of 8 int64, host and device
+// copies; descs / descs_dev: the descriptor table (n_descs rows), host and
+// device copies.  Each step must have rank <= 12 operands and results, at most
+// 2^6 summed terms, and fit its arenas (all checked on the host).
+extern "C" int tedq_tn_contract_small_batch(const void* const* bases, const int64_t* strides, const int64_t* lens,
+                                            const int64_t* records, const int64_t* records_dev, int n_records,
+                                            int64_t batch, const int32_t* descs, const int32_t* descs_dev,
+                                            int n_descs, void* stream) {
+  if (bases == nullptr || strides == nullptr || lens == nullptr || records == nullptr || records_dev == nullptr ||
+      descs == nullptr || descs_dev == nullptr || n_records < 0 || n_descs <= 0 || batch <= 0 || batch > 65535) {
+    set_error("tedq_tn_contract_small_batch: bad arguments");
+    return kBadArgument;
+  }
+  if (n_records == 0) return kOk;
+  TNBases tb;
+  for (int i = 0; i < 3; ++i) {
+    if (bases[i] == nullptr || strides[i] < 0 || lens[i] < 0) {
+      set_error("tedq_tn_contract_small_batch: bad arena %d", i);
+      return kBadArgument;
+    }
+    tb.p[i] = static_cast<const float2*>(bases[i]);
+    tb.stride[i] = strides[i];
+  }
+  for (int r = 0; r < n_records; ++r) {
+    const int64_t* rec = records + 8 * (int64_t)r;
+    TNDesc d;
+    if (rec[0] < 0 || rec[0] >= n_descs || parse_desc(descs + rec[0] * (4 + 9 * 32), &d) != 0) {
+      set_error("tedq_tn_contract_small_batch: record %d: bad descriptor row %lld", r, (long long)rec[0]);
+      return kBadArgument;
+    }
+    const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+    const int64_t sa = 1ll << (nH + nM + nK), sb = 1ll << (nH + nN + nK), sc = 1ll << (nH + nM + nN);
+    const bool shapes = nH + nM + nK <= 12 && nH + nN + nK <= 12 && nH + nM + nN <= 12 && nK <= 6 && rec[6] == sc;
+    const bool in_a = rec[1] >= 0 && rec[1] < 3 && rec[2] >= 0 && rec[2] + sa <= lens[rec[1]];
+    const bool in_b = rec[3] >= 0 && rec[3] < 3 && rec[4] >= 0 && rec[4] + sb <= lens[rec[3]];
+    const bool in_c = rec[5] >= 0 && rec[5] + sc <= lens[2];
+    if (!shapes || !in_a || !in_b || !in_c) {
+      set_error("tedq_tn_contract_small_batch: record %d does not fit (shape or arena bounds)", r);
+      return kBadArgument;
+    }
+  }
+  hipLaunchKernelGGL(k_tn_small_multi, dim3((unsigned)n_records, (unsigned)batch), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), tb, const_cast<float2*>(tb.p[2]), records_dev, descs_dev);
+  return check_launch("tedq_tn_contract_small_batch");
+}
+

@@ -239,6 +239,9 @@ class TNExecutor:
         trials = SEARCH_TRIALS if trials is None else trials
         self.n = cc._num_qubits  # pylint: disable=protected-access
         self.max_bytes = max_bytes
+        # forward: run the small steps of each tree level in one launch (tedq_tn_contract_small_batch)
+        self.small_batch = True
+        self._small = {}
         pool = []          # complex128 constants: matrices and constant parameter values
         recs = []
         self._gate_off = {}
@@ -385,7 +388,68 @@ class TNExecutor:
                 return leaves, self.leaf_stride, off
             return buf["consts"], 0, off
         t = vals[tid]
+        if isinstance(t, tuple):            # a small step's result inside the level arena
+            return t
         return t, t.shape[1], 0
+
+    # small steps, batched per tree level ---------------------------------
+    SMALL_RANK = 12     # operands and result
+    SMALL_SUM_BITS = 6  # summed indices
+
+    def _small_plan(self, net):
+        """The forward steps whose operands are leaves or results of such steps, with ranks <= 12 and at
+        most 2^6 summed terms (the gate merges of a path), grouped by tree level: records for
+        tedq_tn_contract_small_batch, level bounds, result offsets in one arena and its length."""
+        key = id(net)
+        if key in self._small:
+            return self._small[key]
+        offs, level, entries, arena = {}, {}, [], 0
+        for st in net.steps:
+            nh, nm, nn, nk = (int(v) for v in self._descs[st.fwd][:4])
+            if (self._gemm.get(st.fwd) is not None or nk > self.SMALL_SUM_BITS
+                    or max(nh + nm + nk, nh + nn + nk, nh + nm + nn) > self.SMALL_RANK
+                    or not all(t < net.n_inputs or t in offs for t in (st.a, st.b))):
+                continue
+            ops = []
+            for t in (st.a, st.b):
+                if t < net.n_inputs:
+                    where, o = net.leaves[t]
+                    ops += [0 if where == "arena" else 1, o]
+                else:
+                    ops += [2, offs[t]]
+            lv = 1 + max(level.get(t, 0) for t in (st.a, st.b))
+            level[st.c] = lv
+            offs[st.c] = arena
+            size = 1 << net.ranks[st.c]
+            entries.append((lv, [st.fwd] + ops + [arena, size, 0]))
+            arena += size
+        entries.sort(key=lambda e: e[0])
+        recs = np.array([r for _, r in entries], dtype=np.int64).reshape(-1, 8)
+        bounds, start = [], 0
+        for i in range(1, len(entries) + 1):
+            if i == len(entries) or entries[i][0] != entries[start][0]:
+                bounds.append((start, i))
+                start = i
+        plan = {"recs": recs, "bounds": bounds, "offs": offs, "len": max(arena, 1), "dev": {}}
+        self._small[key] = plan
+        return plan
+
+    def _small_levels(self, plan, arena, leaves, buf, B, dev):
+        """Run a plan's levels in order, one launch each, results into ``arena`` [B, plan len]."""
+        key = str(dev)
+        if key not in plan["dev"]:
+            plan["dev"][key] = torch.from_numpy(plan["recs"].reshape(-1).copy()).to(dev)
+        drecs = plan["dev"][key]
+        consts = buf["consts"]
+        bases = (ctypes.c_void_p * 3)(leaves.data_ptr(), consts.data_ptr(), arena.data_ptr())
+        strides = (ctypes.c_int64 * 3)(self.leaf_stride, 0, plan["len"])
+        lens = (ctypes.c_int64 * 3)(self.leaf_stride, consts.numel(), plan["len"])
+        hrecs = plan["recs"]
+        for s, e in plan["bounds"]:
+            hip.call("tedq_tn_contract_small_batch", bases, strides, lens,
+                     hrecs[s:e].ctypes.data_as(hip.c_vp), ctypes.c_void_p(drecs.data_ptr() + 64 * s), e - s, B,
+                     self._descs.ctypes.data_as(hip.c_vp), hip.ptr(buf["descs"]), self._descs.shape[0],
+                     hip.stream_handle(dev))
 
     def _contract(self, row, a, b, c, B, dev, buf):
         g = self._gemm.get(row)
@@ -423,7 +487,15 @@ class TNExecutor:
         buf = self._buffers(dev)
         vals = {}
         kept = {}
+        small = self._small_plan(net) if keep is None and self.small_batch else None
+        if small is not None and small["bounds"]:
+            arena = torch.empty((B, small["len"]), dtype=torch.complex64, device=dev)
+            self._small_levels(small, arena, leaves, buf, B, dev)
+            for tid, o in small["offs"].items():
+                vals[tid] = (arena, small["len"], o)
         for st in net.steps:
+            if small is not None and st.c in small["offs"]:
+                continue
             out = torch.empty((B, 1 << net.ranks[st.c]), dtype=torch.complex64, device=dev)
             vals[st.c] = out
             self._contract(st.fwd, self._operand(net, vals, st.a, leaves, buf),
@@ -434,6 +506,9 @@ class TNExecutor:
                         kept[t] = vals[t]
                     del vals[t]
         root = vals[net.root]
+        if isinstance(root, tuple):
+            t, _, o = root
+            root = t[:, o:o + (1 << net.ranks[net.root])].contiguous()
         return root, kept
 
     # ------------------------------------------------------------------

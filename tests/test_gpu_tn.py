@@ -125,6 +125,23 @@ def test_chain_values_and_gradients_vs_oracle():
         assert np.max(np.abs(grad[b].cpu().numpy() - th.grad.numpy())) < 1e-4
 
 
+def test_small_steps_batched_per_level_match_single_launches():
+    """The forward's gate-merging steps run one launch per tree level (tedq_tn_contract_small_batch);
+    the result equals the one-launch-per-step forward to 2e-5 (different summation order only)."""
+    cc = W.disordered_chain(16, 4, 8, tn_mode=True)
+    ex = cc.executor()
+    net = ex.networks[0]
+    plan = ex._small_plan(net)  # pylint: disable=protected-access
+    assert len(plan["offs"]) > len(net.steps) // 2 and len(plan["bounds"]) < len(plan["offs"])
+    rng = np.random.default_rng(5)
+    rows = torch.tensor(W.chain_rows(rng.uniform(-5, 5, size=(6, 16)), 4), device=DEV)
+    with torch.no_grad():
+        batched = ex.measure(ex.run_forward(rows))[0].cpu().numpy()
+        ex.small_batch = False
+        single = ex.measure(ex.run_forward(rows))[0].cpu().numpy()
+    assert np.max(np.abs(batched - single)) < 2e-5
+
+
 def test_chain_tn_equals_state_vector_engine():
     n, steps, obs = 20, 4, 10
     tn = W.disordered_chain(n, steps, obs, tn_mode=True).executor()

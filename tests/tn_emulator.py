@@ -1,7 +1,7 @@
 """CPU emulation of the tensor-network entry points, for host-logic tests.
 
-Replaces the three device calls of ``TNExecutor`` (gate tensors, one
-contraction step, parameter gradients) with numpy code that reads the same
+Replaces the device calls of ``TNExecutor`` (gate tensors, one contraction
+step, a level of batched small steps, parameter gradients) with numpy code that reads the same
 records and descriptors, so the executor's layouts, descriptors, light-cone
 reduction, path and reverse-tree bookkeeping run unchanged on the CPU.  The
 gate matrices come from the oracle; the contraction is a direct sum over the
@@ -112,6 +112,21 @@ def install(ex):
         cv = tc.numpy().reshape(B, sc)
         cv[:, oc:oc + size_c] = out.astype(np.complex64)
 
+    def small_levels(plan, arena, leaves, buf, B, dev):  # pylint: disable=unused-argument
+        """tedq_tn_contract_small_batch: every record of every level, in level order."""
+        consts = buf["consts"].numpy()
+        bases = [leaves.detach().numpy().reshape(B, -1), consts.reshape(1, -1), arena.numpy()]
+        for s, e in plan["bounds"]:
+            for rec in plan["recs"][s:e]:
+                row, ia, oa, ib, ob, oc, size, _ = (int(v) for v in rec)
+                desc = ex._descs[row]  # pylint: disable=protected-access
+                nH, nM, nN, nK = (int(v) for v in desc[:4])
+                A = bases[ia][:, oa:oa + (1 << (nH + nM + nK))].astype(np.complex128)
+                Bv = bases[ib][:, ob:ob + (1 << (nH + nN + nK))].astype(np.complex128)
+                out = np.zeros((B, size), dtype=np.complex128)
+                contract_np(desc, A, Bv, out)
+                bases[2][:, oc:oc + size] = out.astype(np.complex64)
+
     def buffers(device):  # pylint: disable=unused-argument
         return {"consts": torch.from_numpy(ex._consts)}  # pylint: disable=protected-access
 
@@ -129,6 +144,7 @@ def install(ex):
 
     ex._leaves = leaves  # pylint: disable=protected-access
     ex._contract_raw = contract  # pylint: disable=protected-access
+    ex._small_levels = small_levels  # pylint: disable=protected-access
     ex._buffers = buffers  # pylint: disable=protected-access
     ex._param_grad = param_grad  # pylint: disable=protected-access
     return ex
