@@ -311,6 +311,7 @@ def _chain_leg(W, dev):
 
 def _lattice_leg(W, dev):
     out = {}
+    torch.cuda.empty_cache()  # the backward keeps up to ~94 GB of intermediates per network
     t0 = time.perf_counter()
     ex = W.lattice(5, 6, tn_mode=True).executor()
     build_s = time.perf_counter() - t0

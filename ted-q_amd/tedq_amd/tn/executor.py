@@ -234,10 +234,12 @@ class TNExecutor:
     ``specs`` and ``n_params``.
     """
 
-    def __init__(self, cc, trials=None, max_bytes=64 << 30, paths=None):
+    def __init__(self, cc, trials=None, max_bytes=None, paths=None):
         from .path import SEARCH_TRIALS  # pylint: disable=import-outside-toplevel
         trials = SEARCH_TRIALS if trials is None else trials
         self.n = cc._num_qubits  # pylint: disable=protected-access
+        # bytes of intermediates the backward may keep before it checkpoints segments and recomputes
+        # them; None: 40% of the device's memory (115 GB on a 288 GB MI355X), 64 GiB off the GPU
         self.max_bytes = max_bytes
         # forward: run the small steps of each tree level in one launch (tedq_tn_contract_small_batch)
         self.small_batch = True
@@ -537,16 +539,23 @@ class TNExecutor:
         return list(grads)
 
     # ------------------------------------------------------------------
-    def _segments(self, net, B):
+    def _keep_budget(self, dev=None):
+        if self.max_bytes is not None:
+            return self.max_bytes
+        if dev is not None and dev.type == "cuda":
+            return int(0.4 * torch.cuda.get_device_properties(dev).total_memory)
+        return 64 << 30
+
+    def _segments(self, net, B, dev=None):
         total = net.intermediate_elems() * B * 8
-        if total <= self.max_bytes:
+        if total <= self._keep_budget(dev):
             return [list(range(len(net.steps)))]
         seg = max(1, int(math.ceil(math.sqrt(len(net.steps)))))
         return [list(range(s, min(s + seg, len(net.steps)))) for s in range(0, len(net.steps), seg)]
 
     def _backward_network(self, net, leaves, dmats, root_env, grad, B, dev):
         buf = self._buffers(dev)
-        segs = self._segments(net, B)
+        segs = self._segments(net, B, dev)
         seg_of = {}
         for si, seg in enumerate(segs):
             for s in seg:
@@ -650,7 +659,7 @@ class TNExecutor:
 
     def memory_bytes(self, batch, with_grad=True):  # pylint: disable=unused-argument
         peak = max((net.intermediate_elems() for net in self.networks), default=0)
-        return batch * 8 * (self.leaf_stride * 2 + min(peak, self.max_bytes // max(batch * 8, 1)))
+        return batch * 8 * (self.leaf_stride * 2 + min(peak, self._keep_budget() // max(batch * 8, 1)))
 
     def path_report(self):
         return [dict(net.path_info, steps=len(net.steps), tensors=net.n_inputs) for net in self.networks]
