@@ -489,12 +489,14 @@ class TNExecutor:
         buf = self._buffers(dev)
         vals = {}
         kept = {}
-        small = self._small_plan(net) if keep is None and self.small_batch else None
+        small = self._small_plan(net) if self.small_batch else None
         if small is not None and small["bounds"]:
             arena = torch.empty((B, small["len"]), dtype=torch.complex64, device=dev)
             self._small_levels(small, arena, leaves, buf, B, dev)
             for tid, o in small["offs"].items():
                 vals[tid] = (arena, small["len"], o)
+                if keep is not None and tid != net.root and keep(tid):
+                    kept[tid] = vals[tid]
         for st in net.steps:
             if small is not None and st.c in small["offs"]:
                 continue
