@@ -66,14 +66,16 @@ def step_roles(la, lb, lc):
 
 
 def gemm_operand_layouts(la, lb, lc):
-    """For a GEMM-shaped step, operand index lists with the summed indices as the lowest address bits
-    (A as [h, m, k], B as [h, n, k], k in the same order in both) so that the kernel's tile loads read
-    contiguous runs along k; None for other steps."""
+    """For a GEMM-shaped step, operand index lists with the summed indices as the lowest address bits,
+    in the same order in both operands (A's order), the other indices keeping their order, so that the
+    kernel's tile loads read contiguous runs along k; None for other steps.  The result does not depend
+    on C's layout, so a producer can write an operand in it once and for all."""
     r = step_roles(la, lb, lc)
     if not gemm_shaped(len(r["m"]), len(r["n"]), len(r["k"])):
         return None
-    rev = lambda xs: list(reversed(xs))  # noqa: E731  (role bit 0 is the least significant address bit)
-    return rev(r["h"]) + rev(r["m"]) + rev(r["k"]), rev(r["h"]) + rev(r["n"]) + rev(r["k"])
+    ks = list(reversed(r["k"]))  # role bit 0 is the least significant address bit
+    kset = set(ks)
+    return [x for x in la if x not in kset] + ks, [x for x in lb if x not in kset] + ks
 
 
 def make_desc(la, lb, lc):
@@ -178,6 +180,21 @@ class _Network:
                 out = _step_layout(lay[i], lay[j], res)
             lay.append(out)
             needs.append(needs[i] or needs[j])
+            ops = gemm_operand_layouts(lay[i], lay[j], out)
+            if ops is not None:
+                # an operand produced by a GEMM-shaped step is written there in the layout this step
+                # reads (summed indices lowest) instead of being copied into it before the step
+                for tid, want in ((i, ops[0]), (j, ops[1])):
+                    p = tid - self.n_inputs
+                    if p < 0 or lay[tid] == want or tid == self.n_inputs + len(path) - 1:
+                        continue
+                    ps = self.steps[p]
+                    if gemm_operand_layouts(lay[ps.a], lay[ps.b], lay[tid]) is None:
+                        continue
+                    lay[tid] = list(want)
+                    self.steps[p] = _Step(ps.a, ps.b, tid, desc_rows(lay[ps.a], lay[ps.b], lay[tid]),
+                                          desc_rows(lay[tid], lay[ps.b], lay[ps.a]) if needs[ps.a] else -1,
+                                          desc_rows(lay[ps.a], lay[tid], lay[ps.b]) if needs[ps.b] else -1)
             fwd = desc_rows(lay[i], lay[j], out)
             ba = desc_rows(out, lay[j], lay[i]) if needs[i] else -1
             bb = desc_rows(lay[i], out, lay[j]) if needs[j] else -1
