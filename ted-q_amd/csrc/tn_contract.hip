@@ -637,10 +637,14 @@ constexpr int kTile = 64;
 constexpr int kChunk = 16;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// With split_log > 0 the k range is split over 2^split_log workgroups per
+// tile (few tiles, long sums); their partial tiles go to part[] in C's
+// address order and k_tn_split_sum adds them in a fixed order.
 template <bool THREE_M>
 __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, const float2* __restrict__ B,
                                                  float2* __restrict__ C, const int32_t* __restrict__ dd,
-                                                 int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles) {
+                                                 int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles,
+                                                 float2* __restrict__ part, int split_log, int64_t n_out) {
   __shared__ float2 As[kChunk][kTile + 1];
   __shared__ float2 Bs[kChunk][kTile + 1];
   __shared__ uint32_t rowA[kTile], rowC[kTile], colB[kTile], colC[kTile];
@@ -651,7 +655,8 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
   const int t = threadIdx.x;
   const int64_t b = blockIdx.z;
   const uint32_t h = blockIdx.y;
-  const int mt = blockIdx.x % m_tiles, nt = blockIdx.x / m_tiles;
+  const uint32_t split = blockIdx.x & ((1u << split_log) - 1), tile = blockIdx.x >> split_log;
+  const int mt = tile % m_tiles, nt = tile / m_tiles;
   const uint32_t m0 = (uint32_t)mt * kTile, n0 = (uint32_t)nt * kTile;
   const uint32_t msz = 1u << nM, nsz = 1u << nN;
   if (t < kTile) {
@@ -683,8 +688,8 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
       ci[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       c3[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
-  const uint32_t ksz = 1u << nK;
-  for (uint32_t k0 = 0; k0 < ksz; k0 += kChunk) {
+  const uint32_t kseg = (1u << nK) >> split_log, kbeg = split * kseg, ksz = kbeg + kseg;
+  for (uint32_t k0 = kbeg; k0 < ksz; k0 += kChunk) {
     if (t >= 2 * kTile && t < 2 * kTile + kChunk) {
       const uint32_t k = k0 + (t - 2 * kTile);
       ka[t - 2 * kTile] = scat(d, AK, nK, k < ksz ? k : 0);
@@ -767,7 +772,10 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
         if (m0 + mi < msz && n0 + ni < nsz) {
           const float re = THREE_M ? cr[i][j][v] - ci[i][j][v] : cr[i][j][v];
           const float im = THREE_M ? c3[i][j][v] - cr[i][j][v] - ci[i][j][v] : ci[i][j][v];
-          c[rowC[mi] | colC[ni]] = make_float2(re, im);
+          if (split_log == 0)
+            c[rowC[mi] | colC[ni]] = make_float2(re, im);
+          else
+            part[((b << split_log) + split) * n_out + (rowC[mi] | colC[ni])] = make_float2(re, im);
         }
       }
 }
@@ -943,6 +951,21 @@ static int parse_desc(const int32_t* desc, TNDesc* d) {
   return 0;
 }
 
+// complex products as 4 real MFMA products by default; TEDQ_TN_GEMM=3m selects the 3-product
+// (Gauss) form, which issues 25% fewer MFMAs but needs a third accumulator set (132 VGPRs
+// against 90) and measured slower on the 40-qubit chain (191 ms against 170 ms per batch);
+// TEDQ_TN_GEMM=64 keeps every step on 64 x 64 tiles
+static int gemm_form() {
+  static const int form = [] {
+    const char* e = getenv("TEDQ_TN_GEMM");
+    if (e == nullptr) return 0;
+    return e[0] == '3' ? 3 : (e[0] == '6' ? 64 : 0);
+  }();
+  return form;
+}
+
+static int gemm_tile(const TNDesc& d) { return (gemm_form() == 0 && d.nb[1] >= 7 && d.nb[2] >= 7) ? 128 : 64; }
+
 static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
   TNLaunch L;
   const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
@@ -952,6 +975,11 @@ static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
   L.stream = !L.gemm && nK <= 4 && nH + nM + nN >= 12 && (nH + nK + nN <= 8 || nH + nM + nK <= 8);
   L.lpo_log = 0;
   L.split_log = 0;
+  if (L.gemm && gemm_tile(d) == 64) {
+    // few 64 x 64 tiles and a long sum: split k until about 2048 workgroups (segments of >= 64 terms)
+    const int64_t wgs = (((1ll << nM) + 63) / 64) * (((1ll << nN) + 63) / 64) * (1ll << nH) * batch;
+    while ((wgs << L.split_log) < 2048 && L.split_log < 4 && nK - L.split_log > 6) ++L.split_log;
+  }
   if (L.gemm || L.stream) {
     L.env = false;
     return L;
@@ -1013,21 +1041,19 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
   const float2* Bp = static_cast<const float2*>(b);
   float2* Cp = static_cast<float2*>(c);
   if (L.gemm) {
-    // complex products as 4 real MFMA products by default; TEDQ_TN_GEMM=3m selects the 3-product
-    // (Gauss) form, which issues 25% fewer MFMAs but needs a third accumulator set (132 VGPRs
-    // against 90) and measured slower on the 40-qubit chain (191 ms against 170 ms per batch);
-    // TEDQ_TN_GEMM=64 keeps every step on 64 x 64 tiles
-    static const int form = [] {
-      const char* e = getenv("TEDQ_TN_GEMM");
-      if (e == nullptr) return 0;
-      return e[0] == '3' ? 3 : (e[0] == '6' ? 64 : 0);
-    }();
-    const int tile = (form == 0 && d.nb[1] >= 7 && d.nb[2] >= 7) ? kBig : kTile;
+    const int form = gemm_form();
+    const int tile = gemm_tile(d);
+    const int64_t need = L.split_log ? (int64_t)sizeof(float2) * batch * (L.n_out << L.split_log) : 0;
+    if (need > 0 && (workspace == nullptr || workspace_bytes < need)) {
+      set_error("tedq_tn_contract: workspace of %lld bytes required, %lld given", (long long)need,
+                (long long)workspace_bytes);
+      return kBadArgument;
+    }
     const int m_tiles = (int)(((1ll << d.nb[1]) + tile - 1) / tile);
     const int64_t n_tiles = ((1ll << d.nb[2]) + tile - 1) / tile;
     const int64_t gx = m_tiles * n_tiles;
     const int64_t gy = 1ll << d.nb[0];
-    if (gx > 0x7fffffffll || gy > 65535) {
+    if ((gx << L.split_log) > 0x7fffffffll || gy > 65535) {
       set_error("tedq_tn_contract: grid too large (%lld x %lld)", (long long)gx, (long long)gy);
       return kBadArgument;
     }
@@ -1040,12 +1066,19 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
       hipLaunchKernelGGL(k_tn_gemm128<false>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A,
                          Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
     else if (form != 3)
-      hipLaunchKernelGGL(k_tn_gemm<false>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp,
-                         Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
+      hipLaunchKernelGGL(k_tn_gemm<false>, dim3((unsigned)(gx << L.split_log), (unsigned)gy, (unsigned)batch),
+                         dim3(256), 0, st, A, Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles,
+                         static_cast<float2*>(workspace), L.split_log, L.n_out);
     else
-      hipLaunchKernelGGL(k_tn_gemm<true>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A, Bp,
-                         Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
-    return check_launch("tedq_tn_contract(gemm)");
+      hipLaunchKernelGGL(k_tn_gemm<true>, dim3((unsigned)(gx << L.split_log), (unsigned)gy, (unsigned)batch),
+                         dim3(256), 0, st, A, Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles,
+                         static_cast<float2*>(workspace), L.split_log, L.n_out);
+    int rc = check_launch("tedq_tn_contract(gemm)");
+    if (rc != kOk || L.split_log == 0) return rc;
+    const int64_t sb = (L.n_out + 255) / 256;
+    hipLaunchKernelGGL(k_tn_split_sum, dim3((unsigned)sb, (unsigned)batch), dim3(256), 0, st,
+                       static_cast<const float2*>(workspace), Cp, c_bstride, L.split_log, L.n_out);
+    return check_launch("tedq_tn_contract(split sum)");
   }
   if (d.nb[0] == 0 && d.nb[3] == 0 && (d.nb[1] == 0 || d.nb[2] == 0) && d.nb[1] + d.nb[2] >= kPermBits + 2) {
     // a tensor times a scalar: a layout change through LDS tiles (k_tn_permute)

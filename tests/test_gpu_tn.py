@@ -74,6 +74,8 @@ CASES = [
     ("abcdefghijklmn", "", "nmlkjihgfedcba"),
     ("abcdefghijklmn", "", "jklmnabcdefghi"),
     ("", "abcdefghijklm", "dgmbjelahcfik"),
+    # a 64-wide GEMM with few tiles and a long sum (k split over 16 workgroups per tile)
+    ("abcdef" + "ghijklmnop", "ghijklmnop" + "qrstuvw", "abcdefqrstuvw"),
 ]
 
 
