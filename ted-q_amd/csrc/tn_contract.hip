@@ -791,14 +791,15 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
 // both operands per chunk.  KID: the summed indices are the lowest address
 // bits of both operands in ascending order (the layout the host gives GEMM
 // operands), so a chunk's k offsets need no scatter.  Requires nM, nN >= 7
-// and nK >= 4.
+// and nK >= 4.  Splits k over workgroups as k_tn_gemm does.
 constexpr int kBig = 128;
 constexpr int kBigPad = kBig + 16;  // row stride of 288 words: the four k rows one MFMA read touches use both bank halves
 
 template <bool KID>
 __global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A, const float2* __restrict__ B,
                                                     float2* __restrict__ C, const int32_t* __restrict__ dd,
-                                                    int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles) {
+                                                    int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles,
+                                                    float2* __restrict__ part, int split_log, int64_t n_out) {
   __shared__ float2 As2[2][kChunk][kBigPad];
   __shared__ float2 Bs2[2][kChunk][kBigPad];
   __shared__ uint32_t rowC[kBig], colC[kBig];
@@ -808,9 +809,10 @@ __global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A
   const int t = threadIdx.x;
   const int64_t b = blockIdx.z;
   const uint32_t h = blockIdx.y;
-  const int mt = blockIdx.x % m_tiles, nt = blockIdx.x / m_tiles;
+  const uint32_t split = blockIdx.x & ((1u << split_log) - 1), tile = blockIdx.x >> split_log;
+  const int mt = tile % m_tiles, nt = tile / m_tiles;
   const uint32_t m0 = (uint32_t)mt * kBig, n0 = (uint32_t)nt * kBig;
-  const uint32_t ksz = 1u << nK;
+  const uint32_t kseg = (1u << nK) >> split_log, kbeg = split * kseg, kend = kbeg + kseg;
   if (t < kBig)
     rowC[t] = scat(d, CH, nH, h) | scat(d, CM, nM, m0 + t);
   else
@@ -849,7 +851,7 @@ __global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A
   // LDS holds two chunks: chunk i is multiplied from buffer i % 2 while chunk
   // i + 1 (fetched into registers meanwhile) goes into the other buffer, so
   // one barrier per chunk suffices
-  fetch(0);
+  fetch(kbeg);
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     As2[0][kk][r0 + 16 * r] = pa[r];
@@ -857,10 +859,10 @@ __global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A
   }
   __syncthreads();
   int cur = 0;
-  for (uint32_t k0 = 0; k0 < ksz; k0 += kChunk) {
-    // unconditional (the last chunk re-reads chunk 0, unused), so that the
+  for (uint32_t k0 = kbeg; k0 < kend; k0 += kChunk) {
+    // unconditional (the last chunk re-reads the first, unused), so that the
     // loads stay in the same block as, and ahead of, the MFMAs below
-    fetch(k0 + kChunk < ksz ? k0 + kChunk : 0u);
+    fetch(k0 + kChunk < kend ? k0 + kChunk : kbeg);
     asm volatile("" ::: "memory");  // issue the loads before the MFMAs, not after them
     float2(*As)[kBigPad] = As2[cur];
     float2(*Bs)[kBigPad] = Bs2[cur];
@@ -906,8 +908,13 @@ __global__ void __launch_bounds__(256) k_tn_gemm128(const float2* __restrict__ A
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        c[rowC[wm + 16 * i + 4 * lk + v] | colC[wn + 16 * j + lr]] = make_float2(cr[i][j][v], ci[i][j][v]);
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t o = rowC[wm + 16 * i + 4 * lk + v] | colC[wn + 16 * j + lr];
+        if (split_log == 0)
+          c[o] = make_float2(cr[i][j][v], ci[i][j][v]);
+        else
+          part[((b << split_log) + split) * n_out + o] = make_float2(cr[i][j][v], ci[i][j][v]);
+      }
 }
 
 // ---------------------------------------------------------------------------
@@ -975,10 +982,13 @@ static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
   L.stream = !L.gemm && nK <= 4 && nH + nM + nN >= 12 && (nH + nK + nN <= 8 || nH + nM + nK <= 8);
   L.lpo_log = 0;
   L.split_log = 0;
-  if (L.gemm && gemm_tile(d) == 64) {
-    // few 64 x 64 tiles and a long sum: split k until about 2048 workgroups (segments of >= 64 terms)
-    const int64_t wgs = (((1ll << nM) + 63) / 64) * (((1ll << nN) + 63) / 64) * (1ll << nH) * batch;
-    while ((wgs << L.split_log) < 2048 && L.split_log < 4 && nK - L.split_log > 6) ++L.split_log;
+  if (L.gemm) {
+    // fewer tiles than one round of workgroups (512: two per CU) and a long sum: split k until there
+    // are 512 (segments of >= 64 terms); the environment steps of the backward have such shapes
+    // (256 x 128 results summed over 2^17 terms)
+    const int64_t tile = gemm_tile(d);
+    const int64_t wgs = (((1ll << nM) + tile - 1) / tile) * (((1ll << nN) + tile - 1) / tile) * (1ll << nH) * batch;
+    while ((wgs << L.split_log) < 512 && L.split_log < 11 && nK - L.split_log > 6) ++L.split_log;
   }
   if (L.gemm || L.stream) {
     L.env = false;
@@ -1060,11 +1070,13 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
     bool kid = true;
     for (int j = 0; j < d.nb[3]; ++j) kid = kid && d.pos[AK][j] == j && d.pos[BK][j] == j;
     if (tile == kBig && kid)
-      hipLaunchKernelGGL(k_tn_gemm128<true>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A,
-                         Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
+      hipLaunchKernelGGL(k_tn_gemm128<true>, dim3((unsigned)(gx << L.split_log), (unsigned)gy, (unsigned)batch),
+                         dim3(256), 0, st, A, Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles,
+                         static_cast<float2*>(workspace), L.split_log, L.n_out);
     else if (tile == kBig)
-      hipLaunchKernelGGL(k_tn_gemm128<false>, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, st, A,
-                         Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles);
+      hipLaunchKernelGGL(k_tn_gemm128<false>, dim3((unsigned)(gx << L.split_log), (unsigned)gy, (unsigned)batch),
+                         dim3(256), 0, st, A, Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles,
+                         static_cast<float2*>(workspace), L.split_log, L.n_out);
     else if (form != 3)
       hipLaunchKernelGGL(k_tn_gemm<false>, dim3((unsigned)(gx << L.split_log), (unsigned)gy, (unsigned)batch),
                          dim3(256), 0, st, A, Bp, Cp, desc_dev, a_bstride, b_bstride, c_bstride, m_tiles,

@@ -76,6 +76,8 @@ CASES = [
     ("", "abcdefghijklm", "dgmbjelahcfik"),
     # a 64-wide GEMM with few tiles and a long sum (k split over 16 workgroups per tile)
     ("abcdef" + "ghijklmnop", "ghijklmnop" + "qrstuvw", "abcdefqrstuvw"),
+    # one 128x128 tile and a long sum (k split over 64 workgroups)
+    ("abcdefg" + "hijklmnopqrs", "hijklmnopqrs" + "tuvwxyz", "abcdefgtuvwxyz"),
 ]
 
 
