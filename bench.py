@@ -283,6 +283,22 @@ def tn_leg(c4=True, c3=True):
     return out
 
 
+def tn_bound_s(ex, batch, with_grad):
+    """Roofline time of the executed contraction steps: per step the larger of 8 flops per complex
+    multiply-add at the 157.3 TFLOP/s FP32 matrix peak and its operand + result bytes at 8 TB/s
+    (forward steps, plus both environment steps of the backward when ``with_grad``)."""
+    total = 0.0
+    for net in ex.networks:
+        for st in net.steps:
+            rows = [st.fwd] + ([r for r in (st.bwd_a, st.bwd_b) if r >= 0] if with_grad else [])
+            for r in rows:
+                nh, nm, nn, nk = (int(v) for v in ex._descs[r][:4])  # pylint: disable=protected-access
+                flops = 8.0 * batch * 2.0 ** (nh + nm + nn + nk)
+                byts = 8.0 * batch * (2.0 ** (nh + nm + nk) + 2.0 ** (nh + nn + nk) + 2.0 ** (nh + nm + nn))
+                total += max(flops / 157.3e12, byts / 8e12)
+    return total
+
+
 def _chain_leg(W, dev):
     out = {}
     t0 = time.perf_counter()
@@ -305,6 +321,9 @@ def _chain_leg(W, dev):
                       "evals_per_s": 32 / dt, "ms_per_batch": dt * 1e3, "path_build_s": build_s,
                       "steps": info["steps"], "largest_rank": info["largest_rank"], "macs_per_eval": info["macs"],
                       "achieved_TFLOPs": 8.0 * info["macs"] * 32 / dt / 1e12,
+                      "roofline": {"bound_ms": tn_bound_s(ex, 32, False) * 1e3,
+                                   "frac": tn_bound_s(ex, 32, False) / dt,
+                                   "model": "per step max(8 flops/MAC at 157.3 TFLOP/s, bytes at 8 TB/s)"},
                       "mean_value": float(v.mean())}
     return out
 
@@ -331,6 +350,9 @@ def _lattice_leg(W, dev):
                          "steps": sum(r["steps"] for r in rep),
                          "largest_rank": max(r["largest_rank"] for r in rep),
                          "macs_fwd": sum(r["macs"] for r in rep),
+                         "roofline": {"bound_s": tn_bound_s(ex, 1, True), "frac": tn_bound_s(ex, 1, True) / dt,
+                                      "model": "per forward and environment step max(8 flops/MAC at 157.3 "
+                                               "TFLOP/s, bytes at 8 TB/s)"},
                          "cost": float(sum(float(o[0]) for o in outs) / 25),
                          "grad_finite": bool(torch.isfinite(grad).all())}
     return out
