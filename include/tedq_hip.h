@@ -164,6 +164,20 @@ int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b, int64_t b_
 int tedq_tn_contract_small_batch(const void* const* bases, const int64_t* strides, const int64_t* lens,
                                  const int64_t* records, const int64_t* records_dev, int n_records, int64_t batch,
                                  const int32_t* descs, const int32_t* descs_dev, int n_descs, void* stream);
+/* Both environments of a gate step in one pass.  desc: the forward step C = A B
+ * (host + device copies as in tedq_tn_contract) with one operand X large (at
+ * least 2^10 rows) and the other G gate-sized (at most two summed and two new
+ * indices, no kept-shared ones).  From ec = E_C (C's layout) it writes
+ * ex = E_X[p,k] = sum_q E_C[p,q] G[q,k] in X's layout and eg = E_G[q,k] =
+ * sum_p E_C[p,q] X[p,k] in G's layout, reading E_C and X once.  The workspace
+ * query returns the bytes of partial sums needed, or -1 when the step does not
+ * qualify.  Replaces the two environment contractions torch autograd performs
+ * for such a step of tree.contract (pytorch_backend.py:276-354). */
+int64_t tedq_tn_gate_backward_workspace(const int32_t* desc, int64_t batch);
+int tedq_tn_gate_backward(const void* ec, int64_t ec_bstride, const void* x, int64_t x_bstride, const void* g,
+                          int64_t g_bstride, void* ex, int64_t ex_bstride, void* eg, int64_t eg_bstride,
+                          int64_t batch, const int32_t* desc, const int32_t* desc_dev, void* workspace,
+                          int64_t workspace_bytes, void* stream);
 /* Gate tensors of the network's leaves for every parameter row: ket U with
  * axes (out..., in...) or bra U^dagger, and dV/dtheta 2x2 blocks of trainable
  * parameters.  recs: dev int32[16] records {code, src0..2, const_off, nq,

@@ -472,6 +472,121 @@ __global__ void __launch_bounds__(256) k_tn_small_multi(TNBases bases, float2* _
   }
 }
 
+// Backward of a k_tn_gate step C[p, q] = sum_k X[p, k] G[q, k] in one pass:
+// from the result's environment E_C (C's layout) it writes
+// E_X[p, k] = sum_q E_C[p, q] G[q, k] (X's layout) and sums
+// E_G[q, k] = sum_p E_C[p, q] X[p, k] (G's layout), reading E_C and X once
+// (the two environment steps done separately read E_C twice).  Rows as in
+// k_tn_gate; every workgroup leaves its partial E_G in part[] at G's offsets
+// and k_tn_env_sum adds the workgroups in a fixed order.
+template <int NK, int NQ>
+__global__ void __launch_bounds__(256) k_tn_gate_bwd(const float2* __restrict__ EC, const float2* __restrict__ X,
+                                                     const float2* __restrict__ G, float2* __restrict__ EX,
+                                                     float2* __restrict__ part, const int32_t* __restrict__ dd,
+                                                     int64_t ec_bs, int64_t x_bs, int64_t g_bs, int64_t ex_bs,
+                                                     int a_big) {
+  constexpr int KN = 1 << NK, QN = 1 << NQ;
+  __shared__ TNShared d;
+  __shared__ uint32_t midX[4], midC[4];
+  __shared__ float red[4][2 * QN * KN];
+  load_desc(d, dd);
+  const int XP = a_big ? AM : BN, XK = a_big ? AK : BK, GQ = a_big ? BN : AM, GK = a_big ? BK : AK;
+  const int CP = a_big ? CM : CN, CQ = a_big ? CN : CM;
+  const int nP = d.nb[a_big ? 1 : 2];
+  const int t = threadIdx.x;
+  if (t < 4) {
+    uint32_t ox = 0, oc = 0;
+    for (int j = 8; j < 10; ++j) {
+      const uint32_t bit = ((uint32_t)t >> (j - 8)) & 1u;
+      ox |= bit << d.pos[XP][j];
+      oc |= bit << d.pos[CP][j];
+    }
+    midX[t] = ox;
+    midC[t] = oc;
+  }
+  __syncthreads();
+  const int64_t b = blockIdx.y;
+  const float2* ec = EC + b * ec_bs;
+  const float2* x = X + b * x_bs;
+  const float2* g = G + b * g_bs;
+  float2* ex = EX + b * ex_bs;
+  float2 gm[QN][KN];
+  uint32_t xk[KN], cq[QN], go[QN][KN];
+#pragma unroll
+  for (int k = 0; k < KN; ++k) xk[k] = scat(d, XK, NK, k);
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+    cq[q] = scat(d, CQ, NQ, q);
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+      go[q][k] = scat(d, GQ, NQ, q) | scat(d, GK, NK, k);
+      gm[q][k] = g[go[q][k]];
+    }
+  }
+  uint32_t loX = 0, loC = 0, hiX = 0, hiC = 0;
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t bit = ((uint32_t)t >> j) & 1u;
+    loX |= bit << d.pos[XP][j];
+    loC |= bit << d.pos[CP][j];
+  }
+  const uint32_t blk = blockIdx.x;
+  for (int j = 10; j < nP; ++j) {
+    const uint32_t bit = (blk >> (j - 10)) & 1u;
+    hiX |= bit << d.pos[XP][j];
+    hiC |= bit << d.pos[CP][j];
+  }
+  float2 acc[QN][KN];
+#pragma unroll
+  for (int q = 0; q < QN; ++q)
+#pragma unroll
+    for (int k = 0; k < KN; ++k) acc[q][k] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t xr = hiX | midX[j] | loX, cr = hiC | midC[j] | loC;
+    float2 e[QN], v[KN];
+#pragma unroll
+    for (int q = 0; q < QN; ++q) e[q] = ec[cr | cq[q]];
+#pragma unroll
+    for (int k = 0; k < KN; ++k) v[k] = x[xr | xk[k]];
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+      float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        s.x = fmaf(e[q].x, gm[q][k].x, fmaf(-e[q].y, gm[q][k].y, s.x));
+        s.y = fmaf(e[q].x, gm[q][k].y, fmaf(e[q].y, gm[q][k].x, s.y));
+        acc[q][k].x = fmaf(e[q].x, v[k].x, fmaf(-e[q].y, v[k].y, acc[q][k].x));
+        acc[q][k].y = fmaf(e[q].x, v[k].y, fmaf(e[q].y, v[k].x, acc[q][k].y));
+      }
+      ex[xr | xk[k]] = s;
+    }
+  }
+  const int lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int q = 0; q < QN; ++q)
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+      float re = acc[q][k].x, im = acc[q][k].y;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        re += __shfl_xor(re, o, 64);
+        im += __shfl_xor(im, o, 64);
+      }
+      if (lane == 0) {
+        red[w][2 * (q * KN + k)] = re;
+        red[w][2 * (q * KN + k) + 1] = im;
+      }
+    }
+  __syncthreads();
+  if (t < QN * KN) {
+    const int q = t / KN, k = t % KN;
+    float2 s;
+    s.x = (red[0][2 * t] + red[1][2 * t]) + (red[2][2 * t] + red[3][2 * t]);
+    s.y = (red[0][2 * t + 1] + red[1][2 * t + 1]) + (red[2][2 * t + 1] + red[3][2 * t + 1]);
+    part[(b * gridDim.x + blk) * (QN * KN) + go[q][k]] = s;
+  }
+}
+
 __global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restrict__ C, int64_t c_bs, int split_log,
                                int64_t n_out) {
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1218,5 +1333,81 @@ extern "C" int tedq_tn_contract_small_batch(const void* const* bases, const int6
   hipLaunchKernelGGL(k_tn_small_multi, dim3((unsigned)n_records, (unsigned)batch), dim3(256), 0,
                      static_cast<hipStream_t>(stream), tb, const_cast<float2*>(tb.p[2]), records_dev, descs_dev);
   return check_launch("tedq_tn_contract_small_batch");
+}
+
+// Eligibility of a forward step for tedq_tn_gate_backward (the k_tn_gate shape
+// rule) and the launch shape: returns the partial-sum workspace bytes, or -1.
+static int64_t gate_bwd_plan(const TNDesc& d, int64_t batch, int* a_big, int* nk, int* nq, int* np) {
+  const int nH = d.nb[0], nM = d.nb[1], nN = d.nb[2], nK = d.nb[3];
+  if (nH != 0 || nK > 2) return -1;
+  if (nN <= 2 && nM >= 10) {
+    *a_big = 1;
+    *nq = nN;
+    *np = nM;
+  } else if (nM <= 2 && nN >= 10) {
+    *a_big = 0;
+    *nq = nM;
+    *np = nN;
+  } else {
+    return -1;
+  }
+  *nk = nK;
+  return (int64_t)sizeof(float2) * batch * (1ll << (*np - 10)) * (1ll << (nK + *nq));
+}
+
+extern "C" int64_t tedq_tn_gate_backward_workspace(const int32_t* desc, int64_t batch) {
+  TNDesc d;
+  int a_big, nk, nq, np;
+  if (desc == nullptr || batch <= 0 || parse_desc(desc, &d) != 0) return -1;
+  return gate_bwd_plan(d, batch, &a_big, &nk, &nq, &np);
+}
+
+// Both environments of a gate step (forward descriptor desc: C = A B with one
+// operand X large and the other G gate-sized, as tedq_tn_gate_backward_workspace
+// accepts): ex = E_X in X's layout, eg = E_G in G's layout, from ec = E_C in
+// C's layout; x/g are the step's X and G operands.
+extern "C" int tedq_tn_gate_backward(const void* ec, int64_t ec_bs, const void* x, int64_t x_bs, const void* g,
+                                     int64_t g_bs, void* ex, int64_t ex_bs, void* eg, int64_t eg_bs, int64_t batch,
+                                     const int32_t* desc, const int32_t* desc_dev, void* workspace,
+                                     int64_t workspace_bytes, void* stream) {
+  TNDesc d;
+  int a_big, nk, nq, np;
+  if (ec == nullptr || x == nullptr || g == nullptr || ex == nullptr || eg == nullptr || desc == nullptr ||
+      desc_dev == nullptr || batch <= 0 || batch > 65535 || parse_desc(desc, &d) != 0) {
+    set_error("tedq_tn_gate_backward: bad arguments");
+    return kBadArgument;
+  }
+  const int64_t need = gate_bwd_plan(d, batch, &a_big, &nk, &nq, &np);
+  if (need < 0) {
+    set_error("tedq_tn_gate_backward: the step is not a gate on a large tensor");
+    return kBadArgument;
+  }
+  if (workspace == nullptr || workspace_bytes < need) {
+    set_error("tedq_tn_gate_backward: workspace of %lld bytes required, %lld given", (long long)need,
+              (long long)workspace_bytes);
+    return kBadArgument;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float2* EC = static_cast<const float2*>(ec);
+  const float2* X = static_cast<const float2*>(x);
+  const float2* G = static_cast<const float2*>(g);
+  float2* EX = static_cast<float2*>(ex);
+  float2* part = static_cast<float2*>(workspace);
+  const int split_log = np - 10;
+  const dim3 grid(1u << split_log, (unsigned)batch);
+#define TEDQ_GBWD_CASE(K, Q)                                                                                   \
+  else if (nk == K && nq == Q) hipLaunchKernelGGL((k_tn_gate_bwd<K, Q>), grid, dim3(256), 0, st, EC, X, G, EX, part, \
+                                                  desc_dev, ec_bs, x_bs, g_bs, ex_bs, a_big);
+  if (false) {
+  }
+  TEDQ_GBWD_CASE(0, 0) TEDQ_GBWD_CASE(0, 1) TEDQ_GBWD_CASE(0, 2)
+  TEDQ_GBWD_CASE(1, 0) TEDQ_GBWD_CASE(1, 1) TEDQ_GBWD_CASE(1, 2)
+  TEDQ_GBWD_CASE(2, 0) TEDQ_GBWD_CASE(2, 1) TEDQ_GBWD_CASE(2, 2)
+#undef TEDQ_GBWD_CASE
+  int rc = check_launch("tedq_tn_gate_backward");
+  if (rc != kOk) return rc;
+  hipLaunchKernelGGL(k_tn_env_sum, dim3(1u << (nk + nq), (unsigned)batch), dim3(256), 0, st, part,
+                     static_cast<float2*>(eg), eg_bs, split_log, (int64_t)(1ll << (nk + nq)));
+  return check_launch("tedq_tn_gate_backward(sum)");
 }
 

@@ -45,6 +45,9 @@ _SIGNATURES = {
     "tedq_tn_workspace_bytes": ([c_vp, c_i64], c_i64),
     "tedq_tn_contract": ([c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp], c_int),
     "tedq_tn_contract_small_batch": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_int, c_vp], c_int),
+    "tedq_tn_gate_backward_workspace": ([c_vp, c_i64], c_i64),
+    "tedq_tn_gate_backward": ([c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp,
+                               c_vp, c_i64, c_vp], c_int),
     "tedq_tn_gate_tensors": ([c_vp, c_int, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp],
                              c_int),
     "tedq_tn_param_grad": ([c_vp, c_i64, c_vp, c_i64, c_vp, c_int, c_vp, c_vp, c_i64, c_vp], c_int),

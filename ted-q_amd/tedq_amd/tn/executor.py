@@ -52,6 +52,14 @@ def _bitpos(lst):
     return {s: r - 1 - i for i, s in enumerate(lst)}
 
 
+def _gate_step(desc):
+    """Steps whose backward runs as one pass (tedq_tn_gate_backward_workspace's rule, tn_contract.hip):
+    no kept-shared index, at most two summed indices, one operand gate-sized (at most two new indices)
+    and the other at least 2^10 rows."""
+    nh, nm, nn, nk = (int(v) for v in desc[:4])
+    return nh == 0 and nk <= 2 and ((nn <= 2 and nm >= 10) or (nm <= 2 and nn >= 10))
+
+
 def gemm_shaped(nm, nn, nk):
     """Steps the library runs on the matrix-core GEMM kernel (tn_contract.hip, plan_launch)."""
     return nm >= 6 and nn >= 6 and nk >= 4
@@ -260,6 +268,8 @@ class TNExecutor:
         self.max_bytes = max_bytes
         # forward: run the small steps of each tree level in one launch (tedq_tn_contract_small_batch)
         self.small_batch = True
+        # backward: both environments of a gate step from one pass (tedq_tn_gate_backward)
+        self.fused_backward = True
         self._small = {}
         pool = []          # complex128 constants: matrices and constant parameter values
         recs = []
@@ -609,6 +619,7 @@ class TNExecutor:
                 if ec is None:
                     continue
                 ecop = (ec, ec.shape[1], 0)
+                dsts = {}
                 for which, desc_row, tid in (("a", st.bwd_a, st.a), ("b", st.bwd_b, st.b)):
                     if desc_row < 0:
                         continue
@@ -616,15 +627,20 @@ class TNExecutor:
                         where, off = net.leaves[tid]
                         if where != "arena":
                             continue
-                        dst = (env_arena, self.leaf_stride, off)
+                        dsts[which] = (env_arena, self.leaf_stride, off)
                     else:
                         e = torch.empty((B, 1 << net.ranks[tid]), dtype=torch.complex64, device=dev)
                         envs[tid] = e
-                        dst = (e, e.shape[1], 0)
-                    if which == "a":
-                        self._contract(desc_row, ecop, self._operand(net, vals, st.b, leaves, buf), dst, B, dev, buf)
-                    else:
-                        self._contract(desc_row, self._operand(net, vals, st.a, leaves, buf), ecop, dst, B, dev, buf)
+                        dsts[which] = (e, e.shape[1], 0)
+                aop = self._operand(net, vals, st.a, leaves, buf)
+                bop = self._operand(net, vals, st.b, leaves, buf)
+                if len(dsts) == 2 and self.fused_backward and _gate_step(self._descs[st.fwd]):
+                    self._gate_backward(st, ecop, aop, bop, dsts["a"], dsts["b"], B, dev, buf)
+                    dsts = {}
+                if "a" in dsts:
+                    self._contract(st.bwd_a, ecop, bop, dsts["a"], B, dev, buf)
+                if "b" in dsts:
+                    self._contract(st.bwd_b, aop, ecop, dsts["b"], B, dev, buf)
                 if st.c in vals and st.c != net.root:
                     del vals[st.c]
             for s in seg:
@@ -632,6 +648,24 @@ class TNExecutor:
         if self.n_params:
             self._param_grad(env_arena, dmats, grad, B)
         return root
+
+    def _gate_backward(self, st, ecop, aop, bop, dst_a, dst_b, B, dev, buf):
+        """Both environments of a gate step from one read of E_C (tedq_tn_gate_backward)."""
+        desc = self._descs[st.fwd]
+        a_big = int(desc[1]) >= 10
+        x, g, dx, dg = (aop, bop, dst_a, dst_b) if a_big else (bop, aop, dst_b, dst_a)
+        lib = hip.load()
+        hdesc = desc.ctypes.data_as(hip.c_vp)
+        ws_bytes = lib.tedq_tn_gate_backward_workspace(hdesc, B)
+        if ws_bytes < 0:
+            raise hip.EngineError("tensor network: step is not a gate step")
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        ddesc = ctypes.c_void_p(buf["descs"].data_ptr() + st.fwd * _DESC_WORDS * 4)
+
+        def p(op):
+            return ctypes.c_void_p(op[0].data_ptr() + 8 * op[2]), op[1]
+        hip.call("tedq_tn_gate_backward", *p(ecop), *p(x), *p(g), *p(dx), *p(dg), B, hdesc, ddesc, hip.ptr(ws),
+                 ws_bytes, hip.stream_handle(dev))
 
     def _param_grad(self, env_arena, dmats, grad, B):
         dev = env_arena.device

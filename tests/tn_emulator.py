@@ -1,7 +1,8 @@
 """CPU emulation of the tensor-network entry points, for host-logic tests.
 
 Replaces the device calls of ``TNExecutor`` (gate tensors, one contraction
-step, a level of batched small steps, parameter gradients) with numpy code that reads the same
+step, a level of batched small steps, a fused gate backward, parameter
+gradients) with numpy code that reads the same
 records and descriptors, so the executor's layouts, descriptors, light-cone
 reduction, path and reverse-tree bookkeeping run unchanged on the CPU.  The
 gate matrices come from the oracle; the contraction is a direct sum over the
@@ -145,6 +146,13 @@ def install(ex):
     ex._leaves = leaves  # pylint: disable=protected-access
     ex._contract_raw = contract  # pylint: disable=protected-access
     ex._small_levels = small_levels  # pylint: disable=protected-access
+
+    def gate_backward(st, ecop, aop, bop, dst_a, dst_b, B, dev, buf):
+        """tedq_tn_gate_backward as its two environment steps."""
+        contract(st.bwd_a, ecop, bop, dst_a, B, dev, buf)
+        contract(st.bwd_b, aop, ecop, dst_b, B, dev, buf)
+
+    ex._gate_backward = gate_backward  # pylint: disable=protected-access
     ex._buffers = buffers  # pylint: disable=protected-access
     ex._param_grad = param_grad  # pylint: disable=protected-access
     return ex
