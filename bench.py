@@ -205,6 +205,44 @@ def hea_z0_circuit(n, layers):
     return qai.Circuit(circuit_def, n, *([0.1] * (2 * n * layers)))
 
 
+def small_circuit_leg(dev, reps=50):
+    """The reference's CPU-runnable case: 8-qubit ansatz, 4 layers, <Z_0>, one parameter row, fwd + grad,
+    on the engine (one compiled call per evaluation) and in the oracle's complex64 restatement on the
+    host (the reference algorithm, run in full)."""
+    from oracle import sv_oracle as O  # pylint: disable=import-outside-toplevel
+    n, L = 8, 4
+    theta = np.random.default_rng(0).uniform(0, 2 * np.pi, size=(L, n, 2))
+    compiled = hea_z0_circuit(n, L).compilecircuit(backend="pytorch")
+    t = torch.tensor(theta.reshape(-1), dtype=torch.float32, device=dev)
+
+    def gpu_eval():
+        p = t.detach().requires_grad_(True)
+        compiled(p).sum().backward()
+        return p.grad
+    for _ in range(5):
+        gpu_eval()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g_gpu = gpu_eval()
+    torch.cuda.synchronize()
+    gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+
+    def cpu_eval():
+        p = torch.tensor(theta, dtype=torch.float32, requires_grad=True)
+        out = O.execute(n, O.hea_gates(n, p)[0], [("expval", ("PauliZ", [0]))], cdtype=torch.complex64)
+        out.sum().backward()
+        return p.grad
+    cpu_eval()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g_cpu = cpu_eval()
+    cpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    diff = float((g_gpu.cpu().double() - g_cpu.reshape(-1).double()).abs().max())
+    return {"workload": "hardware-efficient ansatz 8q 4 layers, <Z_0>, one parameter row, fwd + grad",
+            "engine_ms_per_eval": gpu_ms, "cpu_port_ms_per_eval": cpu_ms, "grad_max_diff_vs_cpu_c64": diff}
+
+
 def serving_stream(n_req, seed=4, rate=None):
     """Requests of the mixed stream: n ~ U{6..24}, layers ~ U{2..20}, params ~ U[0, 2pi)."""
     from tedq_amd.serving.engine import Request  # pylint: disable=import-outside-toplevel
@@ -510,6 +548,12 @@ def main():
         check = (idx, vals.detach().double().cpu().numpy(), r.grad.double().cpu().numpy())
         line["cpu_baseline"] = cpu_baseline(n, L, theta[:32], check=check)
         progress(f"cpu baseline: {line['cpu_baseline']['value']:.3f} evals/s")
+        try:
+            line["cpu_baseline"]["small_circuit"] = sc = small_circuit_leg(dev)
+            progress(f"small circuit: engine {sc['engine_ms_per_eval']:.2f} ms, "
+                     f"cpu {sc['cpu_port_ms_per_eval']:.2f} ms per eval")
+        except (RuntimeError, ValueError) as e:  # a side record: never lose the main line for it
+            line["cpu_baseline"]["small_circuit"] = {"error": str(e)[:300]}
     print(json.dumps(line), flush=True)
 
 
