@@ -38,6 +38,45 @@ def test_library_rejects_bad_arguments_without_gpu_work():
     assert lib.tedq_sv_reduce_chunks(20) == 64
 
 
+@pytest.mark.skipif(not os.path.exists(hip.library_path()), reason="library not built")
+def test_contraction_entry_points_validate_on_the_host():
+    """Launch planning and argument checks of the contraction entry points run on the host: workspace
+    sizes of split and unsplit steps, the gate-backward eligibility rule, and records of a small-step
+    batch that leave their arenas are rejected before any launch."""
+    import numpy as np
+    from tedq_amd.tn.executor import _gate_step, make_desc
+    lib = hip.load()
+
+    def desc(la, lb, lc):
+        return make_desc(list(la), list(lb), list(lc))
+
+    def ptr(a):
+        return a.ctypes.data_as(hip.c_vp)
+    # gate environment summed over 2^16 terms: 2^11 workgroup partials of 16 outputs
+    d = desc("ab" + "cdefghijklmnopqr", "st" + "cdefghijklmnopqr", "abst")
+    assert lib.tedq_tn_workspace_bytes(ptr(d), 2) == 8 * 2 * 16 << 4
+    # a one-tile 128x128 GEMM with a 2^12 sum splits k 64 ways; a wide one does not split
+    d = desc("abcdefg" + "hijklmnopqrs", "hijklmnopqrs" + "tuvwxyz", "abcdefgtuvwxyz")
+    assert lib.tedq_tn_workspace_bytes(ptr(d), 3) == 8 * 3 * (1 << 14) * 64
+    d = desc("abcdefghij" + "klmn", "klmn" + "opqrstuvwx", "abcdefghijopqrstuvwx")
+    assert lib.tedq_tn_workspace_bytes(ptr(d), 32) == 0
+    # the fused gate backward accepts gates on >= 2^10 rows, and the Python rule agrees
+    g = desc("abcdefghijkl" + "xy", "xyuv", "abcdefghijkl" + "uv")
+    assert lib.tedq_tn_gate_backward_workspace(ptr(g), 1) == 8 * (1 << 2) * 16 and _gate_step(g)
+    ng = desc("abcdefgh" + "xy", "xyuv", "abcdefgh" + "uv")
+    assert lib.tedq_tn_gate_backward_workspace(ptr(ng), 1) == -1 and not _gate_step(ng)
+    # a small-step record whose result leaves arena 2 is refused without a launch
+    d = desc("ab", "bc", "ac")
+    descs = np.ascontiguousarray(d[None, :])
+    recs = np.array([[0, 0, 0, 0, 4, 2, 4, 0]], dtype=np.int64)       # result at offset 2, 4 elements
+    fake = ctypes.c_void_p(16)                                           # never dereferenced on the host
+    bases = (ctypes.c_void_p * 3)(16, 16, 16)
+    strides = (ctypes.c_int64 * 3)(8, 0, 4)
+    lens = (ctypes.c_int64 * 3)(8, 8, 4)                                 # arena 2 holds 4 elements only
+    rc = lib.tedq_tn_contract_small_batch(bases, strides, lens, ptr(recs), fake, 1, 1, ptr(descs), fake, 1, None)
+    assert rc != 0 and b"does not fit" in lib.tedq_last_error()
+
+
 def test_generated_pass_kernels_compile_for_gfx950(tmp_path):
     """The per-pass kernel generator emits HIP that hipcc accepts for gfx950 (device code only)."""
     import shutil
