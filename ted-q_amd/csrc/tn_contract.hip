@@ -1102,12 +1102,13 @@ static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
   L.lpo_log = 0;
   L.split_log = 0;
   if (L.gemm) {
-    // fewer tiles than one round of workgroups (512: two per CU) and a long sum: split k until there
-    // are 512 (segments of >= 64 terms); the environment steps of the backward have such shapes
-    // (256 x 128 results summed over 2^17 terms)
+    // fewer tiles than about one round of workgroups (the 128 x 128 kernel holds two per CU, the
+    // 64 x 64 one five) and a long sum: split k until there are that many (segments of >= 64 terms);
+    // the environment steps of the backward have such shapes (256 x 128 results summed over 2^17)
     const int64_t tile = gemm_tile(d);
+    const int64_t round = tile == 128 ? 512 : 2048;
     const int64_t wgs = (((1ll << nM) + tile - 1) / tile) * (((1ll << nN) + tile - 1) / tile) * (1ll << nH) * batch;
-    while ((wgs << L.split_log) < 512 && L.split_log < 11 && nK - L.split_log > 6) ++L.split_log;
+    while ((wgs << L.split_log) < round && L.split_log < 11 && nK - L.split_log > 6) ++L.split_log;
   }
   if (L.gemm || L.stream) {
     L.env = false;
