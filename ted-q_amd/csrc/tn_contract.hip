@@ -33,7 +33,12 @@
 //     complex products on the FP32 matrix cores (four real 16x16x4 MFMAs per
 //     complex block and k-step); used when m, n and k are all wide;
 //   * k_tn_gemm128: the same on 128x128 tiles with register prefetch of the
-//     next k chunk, for steps whose m and n are both at least 128 wide.
+//     next k chunk, for steps whose m and n are both at least 128 wide (both
+//     GEMM kernels split long sums over workgroups when tiles are few);
+//   * k_tn_small_multi (tedq_tn_contract_small_batch): every small step of
+//     one level of the contraction tree in one launch;
+//   * k_tn_gate_bwd (tedq_tn_gate_backward): both environments of a
+//     k_tn_gate step from one read of the result's environment.
 // Arithmetic is complex64 with fp32 accumulation, like the reference's
 // complex64 torch contraction.
 #include <stdlib.h>
