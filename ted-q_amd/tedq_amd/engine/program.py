@@ -35,7 +35,9 @@ class EncodedPlan:
 
 def lds_swizzle(e):
     """LDS element swizzle of the pass kernels (csrc/tedq_common.h ``swz``); an XOR-linear involution."""
-    return e ^ (((e >> 4) ^ (e >> 8)) & 15)
+    x = (e >> 4) & 15
+    lo = (x & 6) | ((x ^ (x >> 2)) & 1) | ((bin(x).count("1") & 1) << 3)
+    return e ^ (lo ^ ((e >> 8) & 15))
 
 
 def _pdep(v, positions):
