@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
                                                  float2* __restrict__ C, const int32_t* __restrict__ dd,
                                                  int64_t a_bs, int64_t b_bs, int64_t c_bs, int m_tiles,
                                                  float2* __restrict__ part, int split_log, int64_t n_out) {
-  __shared__ float2 As[kChunk][kTile + 20];  // 84 complex = 168 words, 40 mod 64 (see kBigPad)
+  __shared__ float2 As[kChunk][kTile + 20];  // row stride 168 words, 8 mod 32 banks (see kBigPad)
   __shared__ float2 Bs[kChunk][kTile + 20];
   __shared__ uint32_t rowA[kTile], rowC[kTile], colB[kTile], colC[kTile];
   __shared__ uint32_t ka[kChunk], kb[kChunk];
@@ -913,10 +913,11 @@ __global__ void __launch_bounds__(256) k_tn_gemm(const float2* __restrict__ A, c
 // operands), so a chunk's k offsets need no scatter.  Requires nM, nN >= 7
 // and nK >= 4.  Splits k over workgroups as k_tn_gemm does.
 constexpr int kBig = 128;
-// LDS row stride of 148 complex = 296 words, 40 mod 64 banks: the tile stores (16 k rows x 4 m per
-// wave) hit 64 distinct banks per half-wave and the MFMA operand reads (4 k rows x 16 m) overlap
-// in at most 4 lanes; a stride of 32 mod 64 (conflict-free reads) made the stores 8-way conflicted
-// and the kernel 20% slower (114 against 138 TFLOP/s on a 4096^3 product)
+// LDS row stride of 148 complex = 296 words.  A 16-lane pass of the tile stores covers the 16 k
+// rows of one m: with the earlier stride of 288 words (0 mod 32 banks) all of them fell on one
+// bank pair and the kernel ran at 114 TFLOP/s on a 4096^3 product; at 296 words (8 mod 32) it runs
+// at 138.  Odd strides (258, 262 words: conflict-free stores and reads in that model) measured
+// within 3% of this one.
 constexpr int kBigPad = kBig + 20;
 
 template <bool KID>
