@@ -482,8 +482,9 @@ __global__ void __launch_bounds__(256) k_tn_small_multi(TNBases bases, float2* _
 // E_X[p, k] = sum_q E_C[p, q] G[q, k] (X's layout) and sums
 // E_G[q, k] = sum_p E_C[p, q] X[p, k] (G's layout), reading E_C and X once
 // (the two environment steps done separately read E_C twice).  Rows as in
-// k_tn_gate; every workgroup leaves its partial E_G in part[] at G's offsets
-// and k_tn_env_sum adds the workgroups in a fixed order.
+// k_tn_gate; every workgroup leaves its partial E_G in part[] (the partials of
+// one element contiguous, elements at G's offsets) and k_tn_env_sum adds the
+// workgroups in a fixed order.
 template <int NK, int NQ>
 __global__ void __launch_bounds__(256) k_tn_gate_bwd(const float2* __restrict__ EC, const float2* __restrict__ X,
                                                      const float2* __restrict__ G, float2* __restrict__ EX,
@@ -588,7 +589,7 @@ __global__ void __launch_bounds__(256) k_tn_gate_bwd(const float2* __restrict__ 
     float2 s;
     s.x = (red[0][2 * t] + red[1][2 * t]) + (red[2][2 * t] + red[3][2 * t]);
     s.y = (red[0][2 * t + 1] + red[1][2 * t + 1]) + (red[2][2 * t + 1] + red[3][2 * t + 1]);
-    part[(b * gridDim.x + blk) * (QN * KN) + go[q][k]] = s;
+    part[(b * (QN * KN) + go[q][k]) * gridDim.x + blk] = s;
   }
 }
 
@@ -618,8 +619,9 @@ __global__ void k_tn_split_sum(const float2* __restrict__ part, float2* __restri
 // thread t takes k = (s << kw) | (it << 8) | t, so the k offsets come from a
 // per-thread part (bits 0..7), an LDS table over the iteration (bits 8..kw-1)
 // and a per-workgroup part.  With one workgroup per batch element
-// (split_log 0) the sums go straight to C; otherwise to part[] in C's address
-// order, and k_tn_env_sum adds the workgroups in a fixed order.
+// (split_log 0) the sums go straight to C; otherwise to part[] (the partials
+// of one output contiguous, outputs in C's address order), and k_tn_env_sum
+// adds the workgroups in a fixed order.
 template <int LA, int LB>
 __global__ void __launch_bounds__(256) k_tn_env(const float2* __restrict__ A, const float2* __restrict__ B,
                                                 float2* __restrict__ C, float2* __restrict__ part,
@@ -720,7 +722,7 @@ __global__ void __launch_bounds__(256) k_tn_env(const float2* __restrict__ A, co
       if (split_log == 0)
         C[b * c_bs + offC[t]] = v;
       else
-        part[((b << split_log) + s) * n_out + offC[t]] = v;
+        part[((b * n_out + offC[t]) << split_log) + s] = v;
     }
   }
 }
@@ -735,8 +737,9 @@ __global__ void __launch_bounds__(256) k_tn_env_sum(const float2* __restrict__ p
   const int64_t o = blockIdx.x, b = blockIdx.y;
   const int S = 1 << split_log;
   float2 acc = make_float2(0.f, 0.f);
+  const float2* p = part + (b * n_out + o) * S;  // one output's partials are contiguous
   for (int s = t; s < S; s += 256) {
-    const float2 v = part[(b * S + s) * n_out + o];
+    const float2 v = p[s];
     acc.x += v.x;
     acc.y += v.y;
   }
