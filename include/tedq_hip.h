@@ -137,7 +137,11 @@ int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, in
  * only in A, n kept only in B, k summed), then for each of A_h A_m A_k B_h B_k
  * B_n C_h C_m C_n (32 slots each) the address bit of every role bit.  Every
  * index has dimension 2, so this fixes both operand layouts and the result
- * layout; no permuted copy is made.  Bytes of device workspace the step needs
+ * layout; no permuted copy is made.  Ranks are at most 30, so the last word
+ * (slot 31 of C_n) is free and selects the kernel form of a GEMM-shaped step:
+ * 0 = four real MFMA products per complex product on 128 x 128 tiles where
+ * both free sizes allow it, 3 = the three-product (Gauss) form, 64 = four
+ * products on 64 x 64 tiles only; any other value is rejected.  Bytes of device workspace the step needs
  * for its split-k partial sums (0 if none), or -1 for an invalid descriptor. */
 int64_t tedq_tn_workspace_bytes(const int32_t* desc, int64_t batch);
 /* One pairwise contraction step for every batch element:

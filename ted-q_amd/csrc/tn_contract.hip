@@ -53,7 +53,18 @@ constexpr int kMaxRank = 30;  // per-batch tensor of 2^30 complex64 = 8 GiB
 struct TNDesc {
   int nb[4];               // bits of h, m, n, k
   signed char pos[9][32];  // A_h A_m A_k B_h B_k B_n C_h C_m C_n
+  int gemm_form;           // descriptor word kDescForm (see below)
 };
+
+// The last descriptor word (slot 31 of the C_n role, never a role bit since
+// ranks are at most kMaxRank = 30) selects how a GEMM-shaped step runs:
+// 0 = complex products as 4 real MFMA products on 128 x 128 tiles where both
+// free sizes allow it (the default), 3 = the 3-product (Gauss) form on 64 x 64
+// tiles, 64 = 4 products on 64 x 64 tiles only.  The Gauss form issues 25%
+// fewer MFMAs but needs a third accumulator set (132 VGPRs against 90) and
+// measured slower on the 40-qubit chain (191 ms against 170 ms per batch).
+constexpr int kDescWords = 4 + 9 * 32;
+constexpr int kDescForm = kDescWords - 1;
 enum { AH = 0, AM, AK, BH, BK, BN, CH, CM, CN };
 
 // Role layout as the kernels hold it in LDS (copied from the device
@@ -1057,6 +1068,8 @@ struct TNLaunch {
 };
 
 static int parse_desc(const int32_t* desc, TNDesc* d) {
+  d->gemm_form = desc[kDescForm];
+  if (d->gemm_form != 0 && d->gemm_form != 3 && d->gemm_form != 64) return -1;
   for (int r = 0; r < 4; ++r) {
     if (desc[r] < 0 || desc[r] > kMaxRank) return -1;
     d->nb[r] = desc[r];
@@ -1086,20 +1099,7 @@ static int parse_desc(const int32_t* desc, TNDesc* d) {
   return 0;
 }
 
-// complex products as 4 real MFMA products by default; TEDQ_TN_GEMM=3m selects the 3-product
-// (Gauss) form, which issues 25% fewer MFMAs but needs a third accumulator set (132 VGPRs
-// against 90) and measured slower on the 40-qubit chain (191 ms against 170 ms per batch);
-// TEDQ_TN_GEMM=64 keeps every step on 64 x 64 tiles
-static int gemm_form() {
-  static const int form = [] {
-    const char* e = getenv("TEDQ_TN_GEMM");
-    if (e == nullptr) return 0;
-    return e[0] == '3' ? 3 : (e[0] == '6' ? 64 : 0);
-  }();
-  return form;
-}
-
-static int gemm_tile(const TNDesc& d) { return (gemm_form() == 0 && d.nb[1] >= 7 && d.nb[2] >= 7) ? 128 : 64; }
+static int gemm_tile(const TNDesc& d) { return (d.gemm_form == 0 && d.nb[1] >= 7 && d.nb[2] >= 7) ? 128 : 64; }
 
 static TNLaunch plan_launch(const TNDesc& d, int64_t batch) {
   TNLaunch L;
@@ -1180,7 +1180,7 @@ extern "C" int tedq_tn_contract(const void* a, int64_t a_bstride, const void* b,
   const float2* Bp = static_cast<const float2*>(b);
   float2* Cp = static_cast<float2*>(c);
   if (L.gemm) {
-    const int form = gemm_form();
+    const int form = d.gemm_form;
     const int tile = gemm_tile(d);
     const int64_t need = L.split_log ? (int64_t)sizeof(float2) * batch * (L.n_out << L.split_log) : 0;
     if (need > 0 && (workspace == nullptr || workspace_bytes < need)) {

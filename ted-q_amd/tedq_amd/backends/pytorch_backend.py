@@ -130,6 +130,8 @@ class PyTorchBackend(CompiledCircuit):
         # contraction mode: externally found paths, one per measurement (SSA or opt_einsum-style linear)
         self._tn_paths = kwargs.get("tn_paths", None)
         self._reg_bits_bwd = kwargs.get("reg_bits_bwd", None)
+        # contraction mode: kernel form of GEMM-shaped steps ("4m" default, "3m" Gauss, "4m64" small tiles)
+        self._tn_gemm_form = kwargs.get("tn_gemm_form", "4m")
         if self._interface != "pytorch":
             raise ValueError(f"{self._interface}: pytroch_backend only supports pytorch interface!")
         self._device = None
@@ -197,7 +199,7 @@ class PyTorchBackend(CompiledCircuit):
         if self._executor is None:
             if self._tn_mode or self._use_cotengra or self._use_jdopttn:
                 from ..tn.executor import TNExecutor  # pylint: disable=import-outside-toplevel
-                self._executor = TNExecutor(self, paths=self._tn_paths)
+                self._executor = TNExecutor(self, paths=self._tn_paths, gemm_form=self._tn_gemm_form)
             else:
                 n, units, n_params, pool = lower_circuit(self)
                 init = None

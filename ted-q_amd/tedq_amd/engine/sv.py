@@ -97,8 +97,10 @@ class _DeviceBuffers:
         self.const_pool = torch.from_numpy(enc.const_pool.reshape(-1)).to(device)
         self.pgslot = torch.from_numpy(enc.pgslot).to(device)
         self.prog_base = self.prog.data_ptr()
-        # unknown-gate-code flag of the table builder (set once; never cleared inside a graph)
+        # unknown-gate-code flag of the table builder (set once; never cleared inside a graph), read back
+        # after the first table build that is not being captured into a graph
         self.err = torch.zeros(1, dtype=torch.int32, device=device)
+        self.err_checked = False
 
     def pass_ptr(self, word_offset):
         import ctypes  # pylint: disable=import-outside-toplevel
@@ -177,6 +179,9 @@ class _Program:
                  int(params.dtype == torch.float64), B, self.n_params, hip.ptr(buf.slot_recs),
                  len(self.plan.slots), hip.ptr(buf.comp_recs), hip.ptr(buf.const_pool), hip.ptr(mats),
                  self.plan.mat_stride, hip.ptr(amats), hip.ptr(buf.err), hip.stream_handle(dev))
+        if not buf.err_checked and not torch.cuda.is_current_stream_capturing():
+            check_device_flag(buf.err, "tedq_sv_build_tables: a slot record holds an unknown gate code")
+            buf.err_checked = True
         return mats, amats
 
     def forward_passes(self, psi, mats, probe=None):
@@ -452,6 +457,16 @@ class SVExecutor:
             return amps + tables
         partials = batch * max(self.bwd.plan.n_gslots, 1) * self.bwd.partial_rows() * 32
         return 2 * amps + tables + partials
+
+
+def check_device_flag(flag, what):
+    """Raise EngineError if a device error flag (int32, written by a table kernel) is nonzero.
+
+    Reads one int back, so it synchronises the stream once; callers do it on the first uncaptured
+    build of a program only."""
+    code = int(flag.item())
+    if code:
+        raise hip.EngineError(f"{what} (device error flag {code})")
 
 
 def _factored_args(spec, device):  # pylint: disable=unused-argument

@@ -35,7 +35,7 @@ import numpy as np
 import torch
 
 from ..engine import hip
-from ..engine.sv import MeasSpec
+from ..engine.sv import MeasSpec, check_device_flag
 from ..measurement import Expectation, Probability, State
 from .network import cancel_inverse_pairs
 from .path import external_path, path_sets, path_stats, search_path
@@ -86,8 +86,16 @@ def gemm_operand_layouts(la, lb, lc):
     return [x for x in la if x not in kset] + ks, [x for x in lb if x not in kset] + ks
 
 
-def make_desc(la, lb, lc):
-    """Descriptor of C = contract(A, B) for index lists (most significant first) of A, B and C."""
+# GEMM forms of the descriptor's last word (tn_contract.hip, kDescForm): 4 real products on 128 x 128 tiles
+# where the free sizes allow (default), the 3-product Gauss form, or 4 products on 64 x 64 tiles only
+GEMM_FORMS = {"4m": 0, "3m": 3, "4m64": 64}
+
+
+def make_desc(la, lb, lc, gemm_form=0):
+    """Descriptor of C = contract(A, B) for index lists (most significant first) of A, B and C.
+
+    ``gemm_form`` (a value of GEMM_FORMS) goes to the last word and selects the kernel form of a
+    GEMM-shaped step."""
     sa, sb, sc = set(la), set(lb), set(lc)
     h = sa & sb & sc
     m = (sa - sb) & sc
@@ -117,6 +125,9 @@ def make_desc(la, lb, lc):
     put("CH", roles["h"], pc)
     put("CM", roles["m"], pc)
     put("CN", roles["n"], pc)
+    if gemm_form not in GEMM_FORMS.values():
+        raise ValueError(f"unknown GEMM form {gemm_form}")
+    d[_DESC_WORDS - 1] = gemm_form
     return d
 
 
@@ -259,7 +270,7 @@ class TNExecutor:
     ``specs`` and ``n_params``.
     """
 
-    def __init__(self, cc, trials=None, max_bytes=None, paths=None):
+    def __init__(self, cc, trials=None, max_bytes=None, paths=None, gemm_form="4m"):
         from .path import SEARCH_TRIALS  # pylint: disable=import-outside-toplevel
         trials = SEARCH_TRIALS if trials is None else trials
         self.n = cc._num_qubits  # pylint: disable=protected-access
@@ -326,8 +337,10 @@ class TNExecutor:
 
         lists = []
 
+        form = GEMM_FORMS[gemm_form]
+
         def desc_rows(la, lb, lc):
-            descs.append(make_desc(la, lb, lc))
+            descs.append(make_desc(la, lb, lc, form))
             lists.append((list(la), list(lb), list(lc)))
             return len(descs) - 1
 
@@ -395,6 +408,7 @@ class TNExecutor:
                 "consts": torch.from_numpy(self._consts.copy()).to(device),
                 "descs": torch.from_numpy(self._descs.reshape(-1).copy()).to(device),
                 "err": torch.zeros(1, dtype=torch.int32, device=device),
+                "err_checked": False,
             }
         return self._dev[key]
 
@@ -407,6 +421,9 @@ class TNExecutor:
         hip.call("tedq_tn_gate_tensors", hip.ptr(rows), int(rows.dtype == torch.float64), B, self.n_params,
                  hip.ptr(buf["recs"]), self._recs.shape[0], hip.ptr(buf["pool"]), hip.ptr(leaves),
                  self.leaf_stride, hip.ptr(dmats), dmats.shape[1], hip.ptr(buf["err"]), hip.stream_handle(dev))
+        if not buf["err_checked"] and not torch.cuda.is_current_stream_capturing():
+            check_device_flag(buf["err"], "tedq_tn_gate_tensors: a gate record holds an unknown gate code")
+            buf["err_checked"] = True
         return leaves, dmats
 
     def _operand(self, net, vals, tid, leaves, buf):

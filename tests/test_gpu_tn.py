@@ -26,8 +26,8 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda", 0)
 
 
-def _run_step(la, lb, lc, A, B, batch, shared_b=False):
-    desc = make_desc(la, lb, lc)
+def _run_step(la, lb, lc, A, B, batch, shared_b=False, gemm_form=0):
+    desc = make_desc(la, lb, lc, gemm_form)
     dd = torch.from_numpy(desc.copy()).to(DEV)
     lib = hip.load()
     hdesc = desc.ctypes.data_as(hip.c_vp)
@@ -302,27 +302,23 @@ def test_probs_state_and_matrix_observables(kind):
     assert np.max(np.abs(p.grad.cpu().numpy() - th.grad.numpy())) < 1e-4
 
 
-def test_gemm_three_product_form_matches(monkeypatch):
-    """The 3-product (Gauss) complex GEMM, selected with TEDQ_TN_GEMM=3m in a fresh process, gives the
-    same GEMM-shaped contraction as the default 4-product form (2e-5 relative)."""
-    import subprocess
-    import sys
-    code = (
-        "import numpy as np, sys; sys.path[:0] = ['.', 'ted-q_amd', 'tests'];"
-        "from test_gpu_tn import _run_step;"
-        "rng = np.random.default_rng(1);"
-        "la, lb, lc = list('abcdefghijkl'), list('ijklmnopqrst'), list('abcdefghmnopqrst');"
-        "A = rng.normal(size=(2, 4096)) + 1j * rng.normal(size=(2, 4096));"
-        "B = rng.normal(size=(2, 4096)) + 1j * rng.normal(size=(2, 4096));"
-        "got, _ = _run_step(la, lb, lc, A, B, 2);"
-        "ref = [np.einsum('abcdefghijkl,ijklmnopqrst->abcdefghmnopqrst', A[i].reshape([2]*12), B[i].reshape([2]*12)).reshape(-1) for i in range(2)];"
-        "err = max(np.max(np.abs(got[i] - ref[i])) / np.max(np.abs(ref[i])) for i in range(2));"
-        "print(err); assert err < 2e-5"
-    )
-    env = dict(__import__("os").environ, TEDQ_TN_GEMM="3m")
-    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120,
-                         cwd=__import__("os").path.dirname(__import__("os").path.dirname(__file__)))
-    assert out.returncode == 0, out.stderr[-2000:]
+def test_gemm_three_product_form_matches():
+    """The 3-product (Gauss) complex GEMM and the 64 x 64-tile form, selected per step in the descriptor,
+    give the same GEMM-shaped contraction as the default 4-product form and einsum (2e-5 relative)."""
+    from tedq_amd.tn.executor import GEMM_FORMS
+    rng = np.random.default_rng(1)
+    la, lb, lc = list("abcdefghijkl"), list("ijklmnopqrst"), list("abcdefghmnopqrst")
+    A = rng.normal(size=(2, 4096)) + 1j * rng.normal(size=(2, 4096))
+    B = rng.normal(size=(2, 4096)) + 1j * rng.normal(size=(2, 4096))
+    ref = [np.einsum("abcdefghijkl,ijklmnopqrst->abcdefghmnopqrst", A[i].reshape([2] * 12),
+                     B[i].reshape([2] * 12)).reshape(-1) for i in range(2)]
+    got = {}
+    for name, form in GEMM_FORMS.items():
+        got[name], _ = _run_step(la, lb, lc, A, B, 2, gemm_form=form)
+        err = max(np.max(np.abs(got[name][i] - ref[i])) / np.max(np.abs(ref[i])) for i in range(2))
+        print(f"gemm form {name}: relerr {err:.2e}", flush=True)
+        assert err < 2e-5
+    assert np.max(np.abs(got["3m"] - got["4m"])) > 0  # a different kernel really ran
 
 
 def test_fused_gate_backward_matches_two_environment_steps():

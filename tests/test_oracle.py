@@ -7,7 +7,14 @@ Checks, none of which retype the oracle's own formulas:
 * dense brute force: Kronecker-embedded 2^n x 2^n operators (a construction
   independent of the tensordot/permute loop) on random circuits with every
   gate type, including non-adjacent and reversed qubit orders;
-* gradient identities: autograd == parameter shift == central differences.
+* gradient identities: autograd == parameter shift == central differences;
+* every gate matrix by identities that fix it: Pauli algebra on the
+  computational basis, Pauli exponentials (scipy expm) for the rotations,
+  Rot = RZ RY RZ, S^2 = Z, T^2 = S, SX^2 = X, H = (X+Z)/sqrt 2, CZ / CY / SWAP
+  from CNOT, controlled rotations as |0><0| x I + |1><1| x R, and Toffoli /
+  CSWAP as classical logic on all 8 basis states.  The dense brute force above
+  builds its operators from these same matrices, so these checks are what
+  pins them.
 """
 
 import math
@@ -141,3 +148,145 @@ def test_golden_notebook_values_and_mse_gradient():
     loss = torch.nn.functional.mse_loss(ev, torch.tensor(case["mse_target"], dtype=torch.float64))
     loss.backward()
     assert np.max(np.abs(t.grad.numpy() - case["grad_of_mse_printed_4dp"])) < 5e-4
+
+
+# ---------------------------------------------------------------------------
+# Gate matrices pinned by identities the mathematics fixes (no reference number
+# exists for most of them; reference test_pytorch_backend.py:138-360 only
+# compares the reference's numpy and torch copies with each other).  Each check
+# is chosen so that a plausible slip (a swapped parameter, a flipped sign, the
+# control taken as the target) fails it.
+# ---------------------------------------------------------------------------
+
+def _m(name, *params):
+    return O.gate_matrix(name, list(params), torch.complex128).numpy()
+
+
+def _kron(*ms):
+    out = np.eye(1)
+    for m in ms:
+        out = np.kron(out, m)
+    return out
+
+
+_I2 = np.eye(2)
+_P0 = np.diag([1.0, 0.0])
+_P1 = np.diag([0.0, 1.0])
+
+
+def _basis(bits):
+    v = np.zeros(2 ** len(bits))
+    v[int("".join(str(b) for b in bits), 2)] = 1.0
+    return v
+
+
+def test_pauli_and_clifford_matrices():
+    X, Y, Z = _m("PauliX"), _m("PauliY"), _m("PauliZ")
+    # computational basis: Z|0> = |0>, Z|1> = -|1>, X|0> = |1>
+    assert np.allclose(Z @ _basis([0]), _basis([0])) and np.allclose(Z @ _basis([1]), -_basis([1]))
+    assert np.allclose(X @ _basis([0]), _basis([1])) and np.allclose(X @ _basis([1]), _basis([0]))
+    # Pauli algebra fixes Y including its sign: XY = iZ, Y hermitian and involutory
+    assert np.allclose(X @ Y, 1j * Z)
+    for P in (X, Y, Z):
+        assert np.allclose(P, P.conj().T) and np.allclose(P @ P, _I2)
+    assert np.allclose(_m("I"), _I2)
+    assert np.allclose(_m("Hadamard"), (X + Z) / math.sqrt(2.0))
+    S, T, SX = _m("S"), _m("T"), _m("SX")
+    # S = PhaseShift(pi/2), T = PhaseShift(pi/4) (the definitions), hence S^2 = Z, T^2 = S
+    assert np.allclose(S, _m("PhaseShift", math.pi / 2)) and np.allclose(T, _m("PhaseShift", math.pi / 4))
+    assert np.allclose(S @ S, Z) and np.allclose(T @ T, S)
+    # SX is the principal square root of X: SX^2 = X and SX = e^{i pi/4} RX(pi/2)
+    assert np.allclose(SX @ SX, X)
+    assert np.allclose(SX, np.exp(0.25j * math.pi) * _m("RX", math.pi / 2))
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.3, -1.7, 2.9, 5.5])
+def test_rotation_matrices_are_pauli_exponentials(theta):
+    from scipy.linalg import expm
+    X, Y, Z = _m("PauliX"), _m("PauliY"), _m("PauliZ")
+    assert np.allclose(_m("RX", theta), expm(-0.5j * theta * X))
+    assert np.allclose(_m("RY", theta), expm(-0.5j * theta * Y))
+    assert np.allclose(_m("RZ", theta), expm(-0.5j * theta * Z))
+    # PhaseShift(phi) = diag(1, e^{i phi}) = e^{i phi/2} RZ(phi)
+    assert np.allclose(_m("PhaseShift", theta), np.exp(0.5j * theta) * _m("RZ", theta))
+    assert np.allclose(_m("PhaseShift", theta) @ _basis([0]), _basis([0]))
+
+
+@pytest.mark.parametrize("phi,theta,omega", [(0.3, 0.4, 0.5), (-1.1, 2.2, 0.7), (2.5, -0.9, -2.0)])
+def test_rot_is_rz_ry_rz(phi, theta, omega):
+    # qubit.py:1169-1189: Rot(phi, theta, omega) = RZ(omega) RY(theta) RZ(phi), parameters in call order
+    want = _m("RZ", omega) @ _m("RY", theta) @ _m("RZ", phi)
+    assert np.allclose(_m("Rot", phi, theta, omega), want)
+    # a swapped first and last parameter would differ here (RZ(phi) and RZ(omega) do not commute with RY)
+    assert not np.allclose(_m("Rot", omega, theta, phi), want)
+
+
+def test_cnot_control_first_and_two_qubit_identities():
+    CNOT, H, S = _m("CNOT"), _m("Hadamard"), _m("S")
+    # control is the first listed qubit (qubit.py:644): |a,b> -> |a, a xor b>
+    for a in (0, 1):
+        for b in (0, 1):
+            assert np.allclose(CNOT @ _basis([a, b]), _basis([a, a ^ b]))
+    # CZ = (I x H) CNOT (I x H); CY = (I x S) CNOT (I x S^dagger)
+    assert np.allclose(_m("CZ"), _kron(_I2, H) @ CNOT @ _kron(_I2, H))
+    assert np.allclose(_m("CY"), _kron(_I2, S) @ CNOT @ _kron(_I2, S.conj().T))
+    # SWAP = CNOT_01 CNOT_10 CNOT_01, with CNOT_10 the CNOT controlled by the second qubit
+    swap_h = _kron(H, H)
+    cnot10 = swap_h @ CNOT @ swap_h
+    assert np.allclose(_m("SWAP"), CNOT @ cnot10 @ CNOT)
+    for a in (0, 1):
+        for b in (0, 1):
+            assert np.allclose(_m("SWAP") @ _basis([a, b]), _basis([b, a]))
+
+
+@pytest.mark.parametrize("theta", [0.5, -2.3, 4.0])
+def test_controlled_rotations_are_block_diagonal(theta):
+    # qubit.py:1327-1334, 1429-1642: |0><0| x I + |1><1| x R, control first
+    for cname, rname in (("CRX", "RX"), ("CRY", "RY"), ("CRZ", "RZ"), ("ControlledPhaseShift", "PhaseShift")):
+        want = _kron(_P0, _I2) + _kron(_P1, _m(rname, theta))
+        assert np.allclose(_m(cname, theta), want), cname
+        # the target-first embedding differs at these angles, except for the controlled phase, which is
+        # symmetric in its two qubits (diag(1, 1, 1, e^{i phi}) either way)
+        if cname != "ControlledPhaseShift":
+            assert not np.allclose(_m(cname, theta), _kron(_I2, _P0) + _kron(_m(rname, theta), _P1)), cname
+
+
+def test_three_qubit_gates_on_all_basis_states():
+    # Toffoli: controls first two qubits, X on the third; CSWAP: control first, swap the other two
+    for bits in [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]:
+        a, b, c = bits
+        assert np.allclose(_m("Toffoli") @ _basis(bits), _basis([a, b, c ^ (a & b)])), bits
+        want = [a, c, b] if a else [a, b, c]
+        assert np.allclose(_m("CSWAP") @ _basis(bits), _basis(want)), bits
+
+
+def test_every_gate_is_unitary_and_shift_rules_match_derivatives():
+    rng = np.random.default_rng(11)
+    for name in ALL_GATES:
+        params = rng.uniform(-math.pi, math.pi, size=O.PARAM_COUNT.get(name, 0))
+        m = _m(name, *params)
+        assert np.allclose(m @ m.conj().T, np.eye(m.shape[0])), name
+    # the recipes (ops_abc.py:261-280, qubit.py:32-42) reproduce d/dtheta of a matrix element exactly for
+    # the generators with eigenvalues +-1/2 (two terms) and {0, +-1/2} (four terms)
+    for name in ("RX", "RY", "RZ", "PhaseShift", "CRX", "CRY", "CRZ", "ControlledPhaseShift"):
+        th = 0.77
+        h = 1e-6
+        dim = _m(name, th).shape[0]
+        # expectation values are quadratic in U; the recipe is exact for <U^dag O U>, so compare on one
+        O_ = _kron(*([_m("PauliY")] * int(math.log2(dim))))
+        psi = np.ones(dim) / math.sqrt(dim)
+        f = lambda t: np.real(psi.conj() @ _m(name, t).conj().T @ O_ @ _m(name, t) @ psi)  # noqa: E731
+        d = (f(th + h) - f(th - h)) / (2 * h)
+        d_rec = sum(c * f(a * th + s) for c, a, s in O.shift_recipe(name))
+        assert abs(d - d_rec) < 1e-7, name
+
+
+def test_package_operator_matrices_equal_the_pinned_oracle():
+    """The package's gate classes (used for observables and user matrices) carry the same matrices."""
+    import tedq_amd as qai
+    rng = np.random.default_rng(12)
+    for name in ALL_GATES:
+        params = [float(x) for x in rng.uniform(-math.pi, math.pi, size=O.PARAM_COUNT.get(name, 0))]
+        k = 3 if name in ("CSWAP", "Toffoli") else 2 if (name.startswith("C") or name == "SWAP") else 1
+        g = getattr(qai, name)(*params, qubits=list(range(k)), do_queue=False)
+        assert np.allclose(np.asarray(g.matrix, dtype=np.complex128).reshape(2 ** k, 2 ** k), _m(name, *params)), name
