@@ -98,11 +98,23 @@ int tedq_sv_expval_factored(const void* state, int64_t batch, int n_qubits, int 
 int tedq_sv_lambda_factored(const void* psi, void* lam, int64_t batch, int n_qubits, int q, const int* bits,
                             const float* factors, uint64_t zmask, const double* coef, void* stream);
 /* Probabilities: bits == NULL and k == n gives |psi|^2; otherwise marginals over
- * the k <= 10 physical bits listed (host int32, most significant output bit
- * first).  out: dev float [batch][2^k].  Replaces the probs branch
- * (pytorch_backend.py:461-472). */
+ * the k <= 30 physical bits listed (host int32, distinct, most significant
+ * output bit first), summed in fp64 in a fixed order (no atomics).  out: dev
+ * float [batch][2^k]; partial: dev double [batch][tedq_sv_probs_chunks(n, k)][2^k].
+ * Replaces the probs branch (pytorch_backend.py:461-472). */
 int tedq_sv_probs(const void* state, int64_t batch, int n_qubits, int k, const int* bits, double* partial,
                   float* out, void* stream);
+/* Rows of the partial buffer of a k-bit marginal over n qubits (-1 for bad sizes). */
+int64_t tedq_sv_probs_chunks(int n_qubits, int k);
+/* Post-measurement states of ONE circuit: for every outcome y of the k listed
+ * physical bits (host int32, most significant first, as for tedq_sv_probs),
+ * out[y][r] = psi[x(y, r)] / sqrt(probs[y]) with r over the remaining bits in
+ * C order of the remaining qubits.  probs: dev float [2^k], the marginal of the
+ * same bits; out: dev complex64 [2^n].  Replaces the after_state branch of
+ * probs(qubits, after_state=True) (pytorch_backend.py:474-493), from the
+ * forward's state instead of a second evaluation. */
+int tedq_sv_post_measure(const void* psi, int n_qubits, int k, const int* bits, const float* probs, void* out,
+                         void* stream);
 /* lam += g[b][marg(x)] psi_x (gradient seed of a probability output; g: dev float). */
 int tedq_sv_lambda_probs(const void* psi, void* lam, int64_t batch, int n_qubits, int k, const int* bits,
                          const float* g, void* stream);

@@ -361,23 +361,54 @@ struct BitList {
   int bits[32];  // physical positions, most significant output bit first
 };
 
-// per (b, chunk): LDS histogram of 2^k bins (k <= 10) -> partial[b][chunk][bin]
-__global__ void __launch_bounds__(kRedThreads) k_probs_marg_partial(const float2* __restrict__ st, int n, BitList bl,
-                                                                     double* __restrict__ partial, int64_t nchunk) {
-  __shared__ float bins[1024];
-  const int nb = 1 << bl.k;
-  for (int i = threadIdx.x; i < nb; i += kRedThreads) bins[i] = 0.f;
-  __syncthreads();
-  const int64_t b = blockIdx.y, chunk = blockIdx.x;
-  const int64_t csize = 1ll << (n < kChunkLog ? n : kChunkLog);
-  const int64_t base = (b << n) + chunk * csize;
-  for (int64_t i = threadIdx.x; i < csize; i += kRedThreads) {
-    const float2 a = st[base + i];
-    const uint64_t x = (uint64_t)(chunk * csize + i);
-    atomicAdd(&bins[extract_bits(x, bl.bits, bl.k)], a.x * a.x + a.y * a.y);
+// Physical offset of outcome y of the listed bits (bit k-1-i of y goes to bits[i]).
+__device__ __forceinline__ uint64_t deposit_outcome(uint32_t y, const BitList& bl) {
+  uint64_t x = 0;
+  for (int i = 0; i < bl.k; ++i) x |= (uint64_t)((y >> (bl.k - 1 - i)) & 1u) << bl.bits[i];
+  return x;
+}
+
+// Marginal probabilities without atomics, so the summation order is fixed.
+// Block (bin * nrc + rc, b) sums |psi_x|^2 in fp64 over 2^clog values r of the
+// remaining bits (clog = min(n - k, 14), r in rest chunk rc) of outcome bin, by
+// a fixed per-thread stride and a fixed tree, into partial[b][rc][bin].
+__global__ void __launch_bounds__(kRedThreads) k_probs_marg(const float2* __restrict__ st, int n, BitList bl,
+                                                             uint64_t rest_mask, double* __restrict__ partial,
+                                                             int64_t nrc) {
+  __shared__ double red[kRedThreads / 64];
+  const int64_t b = blockIdx.y;
+  const int64_t bin = blockIdx.x / nrc, rc = blockIdx.x - bin * nrc;
+  const int rest = n - bl.k;
+  const int clog = rest < kChunkLog ? rest : kChunkLog;
+  const uint64_t fixed = deposit_outcome((uint32_t)bin, bl);
+  const float2* base = st + (b << n);
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < (1ll << clog); i += kRedThreads) {
+    const float2 a = base[fixed | pdep64((uint64_t)((rc << clog) + i), rest_mask)];
+    acc += (double)a.x * a.x + (double)a.y * a.y;
   }
+  acc = wave_sum_d(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += kRedThreads) partial[(b * nchunk + chunk) * nb + i] = bins[i];
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) s += red[w];
+    partial[(b * nrc + rc) * (1ll << bl.k) + bin] = s;
+  }
+}
+
+// Post-measurement states of one circuit: out[y][r] = psi[x(y, r)] / sqrt(p[y]) for outcome y of
+// the listed bits and r over the remaining bits (C order of the remaining qubits, ascending).
+__global__ void k_post_measure(const float2* __restrict__ psi, int n, BitList bl, uint64_t rest_mask,
+                               const float* __restrict__ p, float2* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1ll << n)) return;
+  const int rest = n - bl.k;
+  const uint32_t y = (uint32_t)(i >> rest);
+  const uint64_t r = (uint64_t)(i & ((1ll << rest) - 1));
+  const float2 a = psi[deposit_outcome(y, bl) | pdep64(r, rest_mask)];
+  const float scale = sqrtf(p[y]);
+  out[i] = make_float2(a.x / scale, a.y / scale);
 }
 
 // lambda_x += g[b][marg(x)] psi_x   (k == n means the full distribution)
@@ -583,10 +614,35 @@ extern "C" int tedq_sv_lambda_factored(const void* psi, void* lam, int64_t batch
   return check_launch("tedq_sv_lambda_factored");
 }
 
+static int make_bitlist(BitList* bl, int n, int k, const int* bits, const char* what) {
+  bl->k = k;
+  uint64_t seen = 0;
+  for (int i = 0; i < 32; ++i) bl->bits[i] = i < k ? bits[i] : 0;
+  for (int i = 0; i < k; ++i) {
+    if (bl->bits[i] < 0 || bl->bits[i] >= n || ((seen >> bl->bits[i]) & 1)) {
+      set_error("%s: bit %d out of range or repeated", what, bl->bits[i]);
+      return kBadArgument;
+    }
+    seen |= 1ull << bl->bits[i];
+  }
+  return kOk;
+}
+
+static uint64_t rest_mask_of(const BitList& bl, int n) {
+  uint64_t m = (n >= 64) ? ~0ull : ((1ull << n) - 1);
+  for (int i = 0; i < bl.k; ++i) m &= ~(1ull << bl.bits[i]);
+  return m;
+}
+
+extern "C" int64_t tedq_sv_probs_chunks(int n_qubits, int k) {
+  if (n_qubits < 1 || n_qubits > 40 || k < 1 || k > n_qubits) return -1;
+  return n_qubits - k > kChunkLog ? (1ll << (n_qubits - k - kChunkLog)) : 1;
+}
+
 extern "C" int tedq_sv_probs(const void* state, int64_t batch, int n_qubits, int k, const int* bits,
                              double* partial, float* out, void* stream) {
   if (state == nullptr || out == nullptr || batch <= 0 || batch > 65535 || n_qubits < 1 || n_qubits > 40 || k < 1 ||
-      k > n_qubits) {
+      k > n_qubits || k > 30) {
     set_error("tedq_sv_probs: bad arguments (k=%d n=%d)", k, n_qubits);
     return kBadArgument;
   }
@@ -597,26 +653,42 @@ extern "C" int tedq_sv_probs(const void* state, int64_t batch, int n_qubits, int
                        out, total);
     return check_launch("tedq_sv_probs(full)");
   }
-  if (k > 10 || bits == nullptr || partial == nullptr) {
-    set_error("tedq_sv_probs: marginals over %d qubits are not supported (max 10)", k);
-    return kUnsupported;
+  if (bits == nullptr || partial == nullptr) {
+    set_error("tedq_sv_probs: marginals need the bit list and a partial buffer");
+    return kBadArgument;
   }
   BitList bl;
-  bl.k = k;
-  for (int i = 0; i < 32; ++i) bl.bits[i] = i < k ? bits[i] : 0;
-  for (int i = 0; i < k; ++i)
-    if (bl.bits[i] < 0 || bl.bits[i] >= n_qubits) {
-      set_error("tedq_sv_probs: bit %d out of range", bl.bits[i]);
-      return kBadArgument;
-    }
-  const int64_t nchunk = tedq_sv_reduce_chunks(n_qubits);
-  hipLaunchKernelGGL(k_probs_marg_partial, dim3((unsigned)nchunk, (unsigned)batch), dim3(kRedThreads), 0, st,
-                     static_cast<const float2*>(state), n_qubits, bl, partial, nchunk);
-  int rc = check_launch("tedq_sv_probs(partial)");
+  int rc = make_bitlist(&bl, n_qubits, k, bits, "tedq_sv_probs");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_sum_partials<float>, dim3(nblocks(batch << k, 128)), dim3(128), 0, st, partial, batch, nchunk,
+  const int64_t nrc = tedq_sv_probs_chunks(n_qubits, k);
+  if ((nrc << k) > 0x7fffffffll) {
+    set_error("tedq_sv_probs: grid too large");
+    return kBadArgument;
+  }
+  hipLaunchKernelGGL(k_probs_marg, dim3((unsigned)(nrc << k), (unsigned)batch), dim3(kRedThreads), 0, st,
+                     static_cast<const float2*>(state), n_qubits, bl, rest_mask_of(bl, n_qubits), partial, nrc);
+  rc = check_launch("tedq_sv_probs(partial)");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sum_partials<float>, dim3(nblocks(batch << k, 128)), dim3(128), 0, st, partial, batch, nrc,
                      1 << k, out);
   return check_launch("tedq_sv_probs(sum)");
+}
+
+extern "C" int tedq_sv_post_measure(const void* psi, int n_qubits, int k, const int* bits, const float* probs,
+                                    void* out, void* stream) {
+  if (psi == nullptr || bits == nullptr || probs == nullptr || out == nullptr || n_qubits < 1 || n_qubits > 40 ||
+      k < 1 || k > n_qubits || k > 30) {
+    set_error("tedq_sv_post_measure: bad arguments (k=%d n=%d)", k, n_qubits);
+    return kBadArgument;
+  }
+  BitList bl;
+  const int rc = make_bitlist(&bl, n_qubits, k, bits, "tedq_sv_post_measure");
+  if (rc) return rc;
+  const int64_t total = 1ll << n_qubits;
+  hipLaunchKernelGGL(k_post_measure, dim3(nblocks(total, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const float2*>(psi), n_qubits, bl, rest_mask_of(bl, n_qubits), probs,
+                     static_cast<float2*>(out));
+  return check_launch("tedq_sv_post_measure");
 }
 
 extern "C" int tedq_sv_lambda_probs(const void* psi, void* lam, int64_t batch, int n_qubits, int k, const int* bits,

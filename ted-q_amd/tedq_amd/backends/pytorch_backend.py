@@ -28,7 +28,7 @@ import torch
 from .compiled_circuit import CompiledCircuit
 from ..engine.lowering import lower_circuit
 from ..engine.sv import SVExecutor
-from ..tools import dec_to_bin, rescale_state
+from ..tools import dec_to_bin
 
 tcomplex = torch.complex64
 
@@ -55,9 +55,11 @@ class _AdjointFunction(torch.autograd.Function):
     """Forward on the fused SV engine; backward is the adjoint sweep kernel."""
 
     @staticmethod
-    def forward(ctx, executor, rows):  # pylint: disable=arguments-differ
+    def forward(ctx, executor, rows, psi_sink=None):  # pylint: disable=arguments-differ
         psi = executor.run_forward(rows)
         outs = executor.measure(psi)
+        if psi_sink is not None:
+            psi_sink.append(psi)
         ctx.executor = executor
         if isinstance(psi, torch.Tensor):
             ctx.save_for_backward(rows, psi)
@@ -80,11 +82,11 @@ class _AdjointFunction(torch.autograd.Function):
         if hasattr(ctx, "run"):
             (rows,) = ctx.saved_tensors
             grad = ex.run_backward(rows, ctx.run, ex.seed_adjoint(ctx.run, grads), grad_dtype=rows.dtype)
-            return None, grad
+            return None, grad, None
         rows, psi = ctx.saved_tensors
         lam = ex.seed_adjoint(psi, grads)
         grad = ex.run_backward(rows, psi.clone(), lam, grad_dtype=rows.dtype)
-        return None, grad
+        return None, grad, None
 
 
 class TorchExecute(torch.autograd.Function):
@@ -243,15 +245,17 @@ class PyTorchBackend(CompiledCircuit):
         dev = _exec_device(self._device)
         rows = _flat_param_row(params, dev)
         ex = self.executor()
+        sink = []
         if torch.is_grad_enabled() and rows.requires_grad:
-            outs = _AdjointFunction.apply(ex, rows)
+            outs = _AdjointFunction.apply(ex, rows, sink)
         else:
             with torch.no_grad():
                 psi = ex.run_forward(rows)
+                sink.append(psi)
                 outs = ex.measure(psi)
                 outs = [o.to(torch.float32) if s.kind in ("zexp", "fexp") else o for s, o in zip(ex.specs, outs)]
         results = [o[0] for o in outs]
-        self._after_state(rows)
+        self._after_state(sink[0])
         out = self._stack(results, False)
         home = self._device if self._device is not None else torch.device("cpu")
         return out.to(home) if out.device != home else out
@@ -318,22 +322,23 @@ class PyTorchBackend(CompiledCircuit):
         return jac
 
     # ------------------------------------------------------------------
-    def _after_state(self, rows):
-        """``probs(qubits, after_state=True)``: post-measurement states (pytorch_backend.py:474-493)."""
+    def _after_state(self, psi):
+        """``probs(qubits, after_state=True)``: post-measurement states (pytorch_backend.py:474-493), taken
+        from the forward's state ``psi`` ([B, 2^n], row 0) by one gather kernel per measurement.  As in the
+        reference, keys are the outcome bit strings in the order of ``qubits`` and each value is the
+        normalised remaining state as a list over its first remaining axis."""
         wanted = [m for m in self._measurements if getattr(m, "after_state", False) and m.qubits is not None]
-        if not wanted or not isinstance(self.executor(), SVExecutor):
+        if not wanted or not isinstance(psi, torch.Tensor):
             return
-        with torch.no_grad():
-            psi = self.executor().run_forward(rows.detach())[0]
         n = self._num_qubits
-        state = psi.reshape([2] * n)
-        for meas in wanted:
-            size = 2 ** len(meas.qubits)
-            keys, vals = [], []
-            for i in range(size):
-                bits = dec_to_bin(i, size)
-                loc = dict(zip(meas.qubits, bits))
-                sel = tuple(loc.get(ix, slice(None)) for ix in range(n))
-                vals.append(rescale_state(state[sel]))
-                keys.append("".join(str(b) for b in bits))
-            self._states_after_measurement = dict(zip(keys, vals))
+        with torch.no_grad():
+            for meas in wanted:
+                qubits = list(meas.qubits)
+                k = len(qubits)
+                states = self.executor().post_measurement_states(psi[0].contiguous(), qubits)
+                keys, vals = [], []
+                for i in range(2 ** k):
+                    sub = states[i].reshape([2] * (n - k)) if n > k else states[i].reshape(())
+                    vals.append([s for s in sub] if n > k else [sub])
+                    keys.append("".join(str(b) for b in dec_to_bin(i, 2 ** k)))
+                self._states_after_measurement = dict(zip(keys, vals))

@@ -33,6 +33,8 @@ _SIGNATURES = {
     "tedq_sv_expval_factored": ([c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp], c_int),
     "tedq_sv_lambda_factored": ([c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_u64, c_vp, c_vp], c_int),
     "tedq_sv_probs": ([c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp], c_int),
+    "tedq_sv_probs_chunks": ([c_int, c_int], c_i64),
+    "tedq_sv_post_measure": ([c_vp, c_int, c_int, c_vp, c_vp, c_vp, c_vp], c_int),
     "tedq_sv_lambda_probs": ([c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp], c_int),
     "tedq_sv_axpy": ([c_vp, c_vp, c_i64, c_int, c_float, c_vp], c_int),
     "tedq_sum_partials": ([c_vp, c_i64, c_i64, c_int, c_vp, c_vp], c_int),

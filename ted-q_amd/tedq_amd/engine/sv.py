@@ -75,11 +75,10 @@ def compile_measurements(n, measurements):
                 specs.append(MeasSpec("fexp", (), zmask=zmask, fbits=bits,
                                       ffactors=np.stack([rest[b] for b in bits])))
         elif meas.return_type is Probability:
-            if meas.qubits is None:
-                specs.append(MeasSpec("probs", tuple([2] * n)))
-            else:
-                qs = sorted(set(meas.qubits))
-                specs.append(MeasSpec("probs", tuple([2] * len(qs)), pbits=[n - 1 - q for q in qs]))
+            qs = sorted(set(meas.qubits)) if meas.qubits is not None else list(range(n))
+            # every qubit in ascending order is the whole distribution (no marginal sum)
+            pbits = [] if len(qs) == n else [n - 1 - q for q in qs]
+            specs.append(MeasSpec("probs", tuple([2] * len(qs)), pbits=pbits))
         elif meas.return_type is State:
             specs.append(MeasSpec("state", tuple([2] * n)))
         else:
@@ -362,7 +361,8 @@ class SVExecutor:
                 out = torch.empty((B, 2 ** k), dtype=torch.float32, device=dev)
                 if spec.pbits:
                     bits = np.ascontiguousarray(spec.pbits, dtype=np.int32)
-                    partial = torch.empty((B, nchunk, 2 ** k), dtype=torch.float64, device=dev)
+                    rows = hip.load().tedq_sv_probs_chunks(self.n, k)
+                    partial = torch.empty((B, rows, 2 ** k), dtype=torch.float64, device=dev)
                     hip.call("tedq_sv_probs", hip.ptr(psi), B, self.n, k, _hptr(bits), hip.ptr(partial),
                              hip.ptr(out), st)
                 else:
@@ -371,6 +371,22 @@ class SVExecutor:
             else:
                 outs.append(psi.view((B,) + spec.shape))
         return outs
+
+    def post_measurement_states(self, psi_row, qubits):
+        """Normalised post-measurement states of one circuit for every outcome of ``qubits`` (caller's
+        order): [2^k, 2^(n-k)] complex64, outcome y in the bit order of ``qubits``, the remaining qubits
+        in ascending order (the reference's after_state, pytorch_backend.py:474-493)."""
+        dev = psi_row.device
+        k = len(qubits)
+        bits = np.ascontiguousarray([self.n - 1 - q for q in qubits], dtype=np.int32)
+        st = hip.stream_handle(dev)
+        rows = hip.load().tedq_sv_probs_chunks(self.n, k)
+        partial = torch.empty((1, rows, 2 ** k), dtype=torch.float64, device=dev)
+        probs = torch.empty((1, 2 ** k), dtype=torch.float32, device=dev)
+        hip.call("tedq_sv_probs", hip.ptr(psi_row), 1, self.n, k, _hptr(bits), hip.ptr(partial), hip.ptr(probs), st)
+        out = torch.empty((2 ** k, 2 ** (self.n - k)), dtype=torch.complex64, device=dev)
+        hip.call("tedq_sv_post_measure", hip.ptr(psi_row), self.n, k, _hptr(bits), hip.ptr(probs), hip.ptr(out), st)
+        return out
 
     # ------------------------------------------------------------------
     def seed_adjoint(self, psi, grads):

@@ -66,7 +66,8 @@ def test_results_alone_equal_results_in_the_mix():
         ref = O.execute(n, O.hea_gates(n, t)[0], [("expval", ("PauliZ", [0]))])
         ref.sum().backward()
         assert abs(float(ref) - float(r.value[0])) <= 1e-5
-        assert np.max(np.abs(t.grad.numpy().reshape(-1) - r.grad)) <= 1e-4
+        g = t.grad.numpy().reshape(-1)
+        assert np.max(np.abs(g - r.grad)) / np.max(np.abs(g)) <= 1e-5
 
 
 def test_budget_limits_inflight_bytes():
@@ -110,3 +111,39 @@ def test_graph_replay_matches_eager_across_replays():
         torch.cuda.synchronize()
         assert np.allclose(got_v.numpy()[:, 0], outs[0].float().cpu().numpy(), atol=1e-6), rep
         assert np.allclose(got_g.numpy(), grad.cpu().numpy(), atol=1e-6), rep
+
+
+def test_large_requests_alone_equal_results_in_the_mix():
+    """Requests at the top of the serving stream's range (20-24 qubits, up to 20 layers) mixed with
+    small ones: each value and gradient equals the request evaluated alone (2e-6), and a 16-qubit
+    request equals the oracle (1e-5 relative)."""
+    rng = np.random.default_rng(12)
+    cache = {}
+
+    def factory(key):
+        if key not in cache:
+            cache[key] = hea_z0(*key)
+        return cache[key]
+
+    keys = [(24, 20), (6, 3), (22, 12), (20, 20), (8, 2), (20, 20), (16, 5), (24, 2), (6, 3), (21, 7)]
+    reqs = [Request(i, k, rng.uniform(0, 2 * np.pi, 2 * k[0] * k[1]).astype(np.float32), float(i) * 1e-3)
+            for i, k in enumerate(keys)]
+    eng = ServingEngine(factory, hbm_budget=16 << 30, max_batch=8, n_streams=3, max_inflight=6)
+    eng.run(reqs)
+    assert len(eng.completed) == len(reqs)
+    dev = torch.device("cuda", 0)
+    for r in reqs:
+        print(f"request {r.rid} n={r.key[0]} layers={r.key[1]}", flush=True)
+        ex = factory(r.key).executor()
+        row = torch.tensor(r.params[None, :], device=dev)
+        outs, grad = ex.value_and_grad(row, [torch.ones(1, device=dev)])
+        assert np.max(np.abs(outs[0].float().cpu().numpy().reshape(-1) - r.value)) <= 2e-6
+        assert np.max(np.abs(grad.cpu().numpy().reshape(-1) - r.grad)) <= 2e-6
+    r = reqs[6]
+    n, L = r.key
+    t = torch.tensor(r.params.reshape(L, n, 2), dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, O.hea_gates(n, t)[0], [("expval", ("PauliZ", [0]))])
+    ref.sum().backward()
+    assert abs(float(ref) - float(r.value[0])) <= 1e-5
+    g = t.grad.numpy().reshape(-1)
+    assert np.max(np.abs(g - r.grad)) / np.max(np.abs(g)) <= 1e-5

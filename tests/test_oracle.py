@@ -290,3 +290,36 @@ def test_package_operator_matrices_equal_the_pinned_oracle():
         k = 3 if name in ("CSWAP", "Toffoli") else 2 if (name.startswith("C") or name == "SWAP") else 1
         g = getattr(qai, name)(*params, qubits=list(range(k)), do_queue=False)
         assert np.allclose(np.asarray(g.matrix, dtype=np.complex128).reshape(2 ** k, 2 ** k), _m(name, *params)), name
+
+
+def test_hea20_fixture_matches_the_oracle():
+    """tests/golden/hea20_d10_rows.json (the GPU parity reference for the benchmarked circuit) still
+    holds the oracle's numbers: values of both rows from a fresh propagation, read two ways (the
+    oracle's tensordot measurement for three qubits and sum_x p_x (1 - 2 x_q) for all), and three
+    gradient entries per row by central differences."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "hea20_d10_rows.json")) as f:
+        gold = json.load(f)
+    n, L = gold["qubits"], gold["layers"]
+    idx = torch.arange(2 ** n)
+
+    def values(params, all_q=True):
+        with torch.no_grad():
+            psi = O.propagate(n, O.hea_gates(n, np.asarray(params).reshape(L, n, 2))[0])
+        p = (psi.abs() ** 2).reshape(-1)
+        z = np.array([float((p * (1 - 2 * ((idx >> (n - 1 - q)) & 1))).sum()) for q in range(n)])
+        return psi, z
+
+    for row in gold["rows"]:
+        psi, z = values(row["params"])
+        assert np.max(np.abs(z - row["values"])) < 1e-12
+        tz = O.measure(psi, n, [("expval", ("PauliZ", [q])) for q in (0, 9, 19)])
+        assert np.allclose([float(t) for t in tz], np.asarray(row["values"])[[0, 9, 19]], atol=1e-12)
+        h = 1e-5
+        for col in (0, 201, 399):
+            tp, tm = np.array(row["params"]), np.array(row["params"])
+            tp[col] += h
+            tm[col] -= h
+            fd = (values(tp)[1].sum() - values(tm)[1].sum()) / (2 * h)
+            assert abs(fd - row["grad_of_sum"][col]) < 1e-7, col
