@@ -136,9 +136,32 @@ def time_plain_gpu(n, layers, theta, device, budget_s=8.0):
         fwd_s += tb - ta
         bwd_s += time.perf_counter() - tb
         done += 1
+        progress(f"plain single-circuit path: call {done} done (forward {tb - ta:.1f} s, "
+                 f"backward {time.perf_counter() - tb:.1f} s)")
         if time.perf_counter() - t0 > budget_s:
             break
     return done / (time.perf_counter() - t0), done, fwd_s / done, bwd_s / done
+
+
+def engine_batch1(compiled, rows, budget_s=5.0):
+    """The engine's own single-circuit rate: one parameter row per execute_batch call (forward +
+    adjoint gradient of sum_i <Z_i>), same compiled program and kernels as the batched run."""
+    def one(i):
+        r = rows[i:i + 1].detach().requires_grad_(True)
+        compiled.execute_batch(r).sum().backward()
+        return r.grad
+    for i in range(3):
+        one(i)
+    torch.cuda.synchronize()
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and done < rows.shape[0]:
+        one(done)
+        torch.cuda.synchronize()
+        done += 1
+    el = time.perf_counter() - t0
+    return {"evals_per_s": done / el, "ms_per_eval": el / done * 1e3, "calls": done,
+            "how": "one row per call, synchronised after each, generated kernels of the batched run"}
 
 
 def cpu_baseline(n, layers, theta, budget_s=15.0, check=None):
@@ -494,6 +517,8 @@ def main():
     assert torch.isfinite(g).all()
     value = B * args.steps / elapsed
     progress(f"timed: {value:.1f} evals/s, {elapsed / args.steps * 1e3:.2f} ms per step")
+    from tedq_amd.engine.sv import norm_deviation  # pylint: disable=import-outside-toplevel
+    norm_dev = norm_deviation(ex)
 
     amps = B * (2 ** n)
     kernels = {}
@@ -523,6 +548,9 @@ def main():
                      "traffic": measured_traffic(dom, n, L, B, ex),
                      "bytes_per_launch": kd["bytes_per_launch"], "mean_launch_ms": kd["mean_ms"]},
         "kernels": kernels,
+        "norm_check": {"max_abs_deviation": norm_dev, "rows": B,
+                       "how": "<psi|psi> of every row of the last timed step, fp64 compensated reduction on "
+                              "the device in the measurement pass"},
         # speed figure: the sweeps an unfused gate-by-gate engine makes per evaluation (3 per gate in the
         # forward + adjoint, one per parameter, one for the measurement), 16 B per amplitude each;
         # fusion lets this exceed HBM bandwidth, so it is never a roofline fraction
@@ -534,11 +562,14 @@ def main():
     if not args.skip_tn:
         line["tensor_network"] = tn_leg(c4=not args.skip_c4)
     if not args.skip_plain:
+        b1 = engine_batch1(compiled, rows)
+        progress(f"engine, one row per call: {b1['evals_per_s']:.1f} evals/s")
         ps, cnt, fwd_s, bwd_s = time_plain_gpu(n, L, theta[:16], dev)
         progress(f"plain single-circuit path: {ps:.4f} evals/s")
         line["plain_gpu"] = {"value": ps, "unit": "evals/s", "sample": f"{cnt} single-row calls",
                              "forward_s": fwd_s, "backward_s": bwd_s,
-                             "speedup_batched_over_plain": value / ps}
+                             "speedup_batched_over_plain": value / ps,
+                             "engine_batch1": b1, "speedup_batched_over_engine_batch1": value / b1["evals_per_s"]}
     if not args.skip_cpu:
         # the engine's values and gradients of two rows of the benchmarked batch, for the error check
         idx = [0, min(B, 32) - 1]

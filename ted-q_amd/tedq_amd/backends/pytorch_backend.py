@@ -20,7 +20,9 @@ reference PyTorch backend (tedq/backends/pytorch_backend.py:44-1223):
 
 Beyond the reference: ``execute_batch(params_2d)`` evaluates B parameter rows
 in one set of launches (shape [B, num_measurements, ...]).  The device is held
-per instance, not per class.
+per instance, not per class.  Every state-vector forward also reduces
+<psi|psi> in fp64 on the device (``last_norm``); ``norm_tol=<float>`` makes
+each call raise when max |<psi|psi> - 1| exceeds it.
 """
 
 import torch
@@ -134,6 +136,8 @@ class PyTorchBackend(CompiledCircuit):
         self._reg_bits_bwd = kwargs.get("reg_bits_bwd", None)
         # contraction mode: kernel form of GEMM-shaped steps ("4m" default, "3m" Gauss, "4m64" small tiles)
         self._tn_gemm_form = kwargs.get("tn_gemm_form", "4m")
+        # state-vector mode: check max |<psi|psi> - 1| (fp64 on the device) after every forward
+        self._norm_tol = kwargs.get("norm_tol", None)
         if self._interface != "pytorch":
             raise ValueError(f"{self._interface}: pytroch_backend only supports pytorch interface!")
         self._device = None
@@ -211,7 +215,16 @@ class PyTorchBackend(CompiledCircuit):
                                             k_max=self._tile_bits, r=self._reg_bits,
                                             k_bwd=self._tile_bits_bwd, r_bwd=self._reg_bits_bwd)
                 self._executor.jit = self._jit
+                self._executor.norm_tol = self._norm_tol
         return self._executor
+
+    @property
+    def last_norm(self):
+        """<psi|psi> of every row of the last state-vector forward, reduced in fp64 on the device
+        (compensated sums); a device float64 tensor [B], or None before the first call / in
+        contraction mode."""
+        ex = self._executor
+        return getattr(ex, "last_norm", None) if ex is not None else None
 
     def _flatten_and_update(self, params):
         views = []

@@ -23,6 +23,7 @@ from .planner import build_plan
 from .program import encode_plan
 
 _Z = np.array([[1, 0], [0, -1]], dtype=np.complex128)
+ZMAX = 32                      # Z-string masks per reduction launch (csrc/sv_measure.hip kMaxZ)
 _I = np.eye(2, dtype=np.complex128)
 
 
@@ -83,8 +84,6 @@ def compile_measurements(n, measurements):
             specs.append(MeasSpec("state", tuple([2] * n)))
         else:
             raise NotImplementedError(f"measurement {meas.return_type} is not supported")
-    if len(zmasks) > 32:
-        raise NotImplementedError("more than 32 Z-string expectation values per circuit")
     return specs, zmasks
 
 
@@ -239,6 +238,12 @@ class SVExecutor:
         self._units = units
         self._pool = list(const_pool)
         self.specs, self.zmasks = compile_measurements(n, measurements)
+        # every forward measurement also reduces <psi|psi> in fp64 (the Z-string of mask 0, compensated
+        # sums), appended after the circuit's own masks; the last batch's values stay in last_norm
+        self._zall = list(self.zmasks) + [0]
+        self.last_norm = None
+        # when set, every measurement checks max |<psi|psi> - 1| <= norm_tol (one read-back per call)
+        self.norm_tol = None
         self.init_state = None
         if init_state is not None:
             flat = np.asarray(init_state, dtype=np.complex64).reshape(-1)
@@ -306,8 +311,7 @@ class SVExecutor:
     def _zmask_buffer(self, device):
         key = (device.type, device.index)
         if key not in self._dev:
-            z = torch.tensor(np.asarray(self.zmasks, dtype=np.uint64).view(np.int64), device=device) \
-                if self.zmasks else None
+            z = torch.tensor(np.asarray(self._zall, dtype=np.uint64).view(np.int64), device=device)
             init = None
             if self.init_state is not None:
                 init = torch.from_numpy(self.init_state.view(np.float32).copy()).to(device)
@@ -338,13 +342,23 @@ class SVExecutor:
         B = psi.shape[0]
         st = hip.stream_handle(dev)
         nchunk = hip.load().tedq_sv_reduce_chunks(self.n)
-        zout = None
-        if self.zmasks:
-            M = len(self.zmasks)
-            partial = torch.empty((B, nchunk, M), dtype=torch.float64, device=dev)
-            zout = torch.empty((B, M), dtype=torch.float64, device=dev)
-            hip.call("tedq_sv_expval_z", hip.ptr(psi), B, self.n, hip.ptr(zbuf), M, hip.ptr(partial),
-                     hip.ptr(zout), st)
+        # all Z-string observables and the norm, at most ZMAX masks per reduction pass
+        M = len(self._zall)
+        zout = torch.empty((B, M), dtype=torch.float64, device=dev)
+        partial = torch.empty((B, nchunk, min(M, ZMAX)), dtype=torch.float64, device=dev)
+        for c0 in range(0, M, ZMAX):
+            m = min(ZMAX, M - c0)
+            part = zout if m == M else torch.empty((B, m), dtype=torch.float64, device=dev)
+            hip.call("tedq_sv_expval_z", hip.ptr(psi), B, self.n, _offset_ptr(zbuf, c0), m, hip.ptr(partial),
+                     hip.ptr(part), st)
+            if m != M:
+                zout[:, c0:c0 + m] = part
+        self.last_norm = zout[:, M - 1]
+        if self.norm_tol is not None and not torch.cuda.is_current_stream_capturing():
+            dev_max = norm_deviation(self)
+            if not dev_max <= self.norm_tol:
+                raise hip.EngineError(f"state norm check failed: max |<psi|psi> - 1| = {dev_max:.3e} over "
+                                      f"{B} rows exceeds {self.norm_tol:.1e}")
         outs = []
         for spec in self.specs:
             if spec.kind == "zexp":
@@ -398,6 +412,7 @@ class SVExecutor:
         lam = torch.empty_like(psi)
         M = len(self.zmasks)
         dy = None
+        cols = []
         if M:
             cols = [None] * M
             for spec, g in zip(self.specs, grads):
@@ -410,9 +425,15 @@ class SVExecutor:
                     if zero is None:
                         zero = torch.full((B,), 0.0, dtype=torch.float64, device=dev)
                     cols[i] = zero
-            dy = torch.stack(cols, 1).contiguous()
-        hip.call("tedq_sv_lambda_z", hip.ptr(psi), hip.ptr(lam), B, self.n, hip.ptr(zbuf) if M else None, M,
+            dy = torch.stack(cols[:ZMAX], 1).contiguous()
+        m0 = min(M, ZMAX)
+        hip.call("tedq_sv_lambda_z", hip.ptr(psi), hip.ptr(lam), B, self.n, hip.ptr(zbuf) if m0 else None, m0,
                  hip.ptr(dy), st)
+        # observables past the first ZMAX: lam += dy_m Z-string_m psi, one accumulating pass each
+        for m in range(ZMAX, M):
+            coef = cols[m].contiguous()
+            hip.call("tedq_sv_lambda_factored", hip.ptr(psi), hip.ptr(lam), B, self.n, 0, None, None,
+                     self.zmasks[m], hip.ptr(coef), st)
         for spec, g in zip(self.specs, grads):
             if g is None or spec.kind == "zexp":
                 continue
@@ -475,6 +496,14 @@ class SVExecutor:
         return 2 * amps + tables + partials
 
 
+def norm_deviation(ex):
+    """Largest |<psi|psi> - 1| over the rows of an executor's last forward measurement (fp64, reduced on
+    the device with compensated sums); None before the first measurement.  Reads the values back."""
+    if ex.last_norm is None:
+        return None
+    return float((ex.last_norm - 1.0).abs().max())
+
+
 def check_device_flag(flag, what):
     """Raise EngineError if a device error flag (int32, written by a table kernel) is nonzero.
 
@@ -490,6 +519,12 @@ def _factored_args(spec, device):  # pylint: disable=unused-argument
     bits = np.ascontiguousarray(spec.fbits, dtype=np.int32)
     fac = np.ascontiguousarray(spec.ffactors.astype(np.complex64)).view(np.float32).reshape(-1)
     return bits, fac
+
+
+def _offset_ptr(t, elems):
+    """Device pointer ``elems`` elements into a contiguous tensor."""
+    import ctypes  # pylint: disable=import-outside-toplevel
+    return ctypes.c_void_p(t.data_ptr() + elems * t.element_size())
 
 
 def _hptr(a):

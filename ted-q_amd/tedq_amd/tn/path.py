@@ -76,8 +76,10 @@ def greedy_path(inputs, output, alpha=1.0, temperature=0.0, seed=0, absorb=0):
         heapq.heappush(heap, (noisy(c), i, j))
 
     seen = set()
-    for ix, hs in holders.items():
-        hs = sorted(hs)
+    # candidate pairs are pushed in index order: with noise, the draw a pair gets depends on the
+    # push order, which must not follow the (per-process) hash order of the index symbols
+    for ix in sorted(holders):
+        hs = sorted(holders[ix])
         for a in range(len(hs)):
             for b in range(a + 1, len(hs)):
                 if (hs[a], hs[b]) not in seen:

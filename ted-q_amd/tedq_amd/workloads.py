@@ -77,9 +77,11 @@ def chain_rows(h, steps=8, dt=0.1):
     return np.tile(2.0 * h * dt, (1, steps)).astype(np.float32)
 
 
-def lattice(L=5, layers=6, **compile_kwargs):
-    """Compiled L x L lattice circuit with one <Z_i> per qubit (L*L outputs)."""
+def lattice(L=5, layers=6, observe=None, **compile_kwargs):
+    """Compiled L x L lattice circuit with one <Z_i> per qubit (L*L outputs), or per qubit of
+    ``observe`` when given."""
     n = L * L
+    observed = list(range(n)) if observe is None else list(observe)
 
     def circuit_def(*p):
         it = iter(p)
@@ -93,6 +95,6 @@ def lattice(L=5, layers=6, **compile_kwargs):
             for r in range(L - 1):
                 for c in range(L):
                     _g.CZ(qubits=[r * L + c, (r + 1) * L + c])
-        return [_m.expval(_g.PauliZ(qubits=[q])) for q in range(n)]
+        return [_m.expval(_g.PauliZ(qubits=[q])) for q in observed]
 
     return Circuit(circuit_def, n, *([0.1] * (2 * n * layers))).compilecircuit(backend="pytorch", **compile_kwargs)

@@ -19,11 +19,15 @@ from tedq_amd.backends.pytorch_backend import PyTorchBackend
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda", 0)
 TOL = 1e-5
+# Gradients of a random circuit can vanish identically (an observable outside every parameter's
+# light cone); their error is then measured against 0.1, a tenth of the largest derivative a Pauli
+# rotation can give a unit-norm observable, instead of against rounding noise.
+GFLOOR = 0.1
 
 
-def relerr(x, ref):
+def relerr(x, ref, floor=1e-12):
     x, ref = np.asarray(x), np.asarray(ref)
-    return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-12))
+    return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), floor))
 
 
 def _golden(meas):
@@ -153,7 +157,7 @@ def test_random_circuits_forward_and_gradient(seed, n, tile):
     ref = O.execute(n, _oracle_gates(spec, list(t64)), [("expval", ("PauliZ", [q])) for q in obs])
     (ref * dy.double().cpu()).sum().backward()
     e_f = relerr(out.detach().cpu().numpy(), ref.detach().numpy())
-    e_g = relerr(theta.grad.cpu().numpy(), t64.grad.numpy())
+    e_g = relerr(theta.grad.cpu().numpy(), t64.grad.numpy(), GFLOOR)
     print(f"  forward relerr {e_f:.2e} gradient relerr {e_g:.2e}", flush=True)
     assert e_f < TOL
     assert e_g < TOL
@@ -187,7 +191,7 @@ def test_state_probs_and_nondiagonal_observables(n):
         rr = O.execute(n, _oracle_gates(spec, list(t64)), [("expval", ob)])
         rr.sum().backward()
         assert abs(r.item() - rr.item()) < 1e-5
-        assert relerr(t.grad.cpu().numpy(), t64.grad.numpy()) < TOL
+        assert relerr(t.grad.cpu().numpy(), t64.grad.numpy(), GFLOOR) < TOL
 
 
 def test_probs_and_state_gradients_composite_cost():
@@ -241,7 +245,7 @@ def test_batch_equals_single_and_c1_config():
         ref = O.execute(n, O.hea_gates(n, t64)[0], [("expval", ("PauliZ", [0]))])
         ref.sum().backward()
         assert relerr(out[i].detach().cpu().numpy(), ref.detach().numpy()) < TOL
-        assert relerr(rows.grad[i].cpu().numpy(), t64.grad.numpy().reshape(-1)) < TOL
+        assert relerr(rows.grad[i].cpu().numpy(), t64.grad.numpy().reshape(-1), GFLOOR) < TOL
 
 
 def test_c2_full_size_properties():
@@ -308,7 +312,7 @@ def test_generated_pass_kernels_match_interpreter_and_oracle(seed, n, tile, reg)
     ref.sum().backward()
     got = np.array([float(v[0]) for v in vj])
     assert relerr(got, ref.detach().numpy()) < TOL
-    assert relerr(gj[0].cpu().numpy(), t64.grad.numpy()) < TOL
+    assert relerr(gj[0].cpu().numpy(), t64.grad.numpy(), GFLOOR) < TOL
 
 
 def test_generated_kernels_on_the_benchmark_circuit():
@@ -488,10 +492,16 @@ def test_probs_after_state_vs_reference_semantics():
         for q, bit in zip(qubits, key):
             sel[q] = int(bit)
         sub = ref[tuple(sel)]
-        want = sub / np.linalg.norm(sub)
         # the reference returns the normalised sub-state as a list over its first remaining axis
         assert len(parts) == 2
         got = np.stack([p.cpu().numpy() for p in parts])
+        bit_of = dict(zip(qubits, key))
+        p_engine = float(probs.cpu().numpy().reshape([2] * len(qubits))[tuple(int(bit_of[q]) for q in sorted(qubits))])
+        if p_engine == 0.0:
+            # an impossible outcome: the reference's rescale divides by sqrt(0) (helpers.py:43-51)
+            assert np.linalg.norm(sub) < 1e-6 and np.isnan(got).all(), key
+            continue
+        want = sub / np.linalg.norm(sub)
         assert got.shape == want.shape and relerr(got, want) < TOL, key
     refp = O.measure(torch.as_tensor(ref), n, [("probs", qubits)])[0].numpy()
     assert relerr(probs.cpu().numpy().reshape(refp.shape), refp) < TOL
@@ -507,3 +517,66 @@ def test_unknown_gate_code_raises():
     buf.comp_recs[0] = 99          # component code word of the first slot
     with pytest.raises(EngineError, match="unknown gate code"):
         ex.run_forward(torch.zeros((1, npar), device=DEV))
+
+
+def test_fp64_norm_check_on_the_device():
+    """Every forward reduces <psi|psi> in fp64 on the device (mask 0 of the Z-string pass, compensated
+    sums): it matches a float64 sum of the returned state, stays within 1e-5 of 1 on the 20-qubit
+    benchmark circuit, and an unnormalised initial vector is reported (norm_tol raises)."""
+    from tedq_amd import workloads as W
+    from tedq_amd.engine.hip import EngineError
+    from tedq_amd.engine.sv import norm_deviation
+    ex = W.hea(20, 10).executor()
+    rows = torch.tensor(np.random.default_rng(1).uniform(0, 2 * np.pi, size=(4, 400)), dtype=torch.float32,
+                        device=DEV)
+    psi = ex.run_forward(rows)
+    ex.measure(psi)
+    direct = psi.to(torch.complex128).abs().pow(2).sum(1)
+    assert float((ex.last_norm - direct).abs().max()) < 1e-12
+    print(f"20q d10 max |<psi|psi> - 1| = {norm_deviation(ex):.2e}", flush=True)
+    assert norm_deviation(ex) < 1e-5
+    n = 6
+    init = np.random.default_rng(2).normal(size=2 ** n) + 0j
+    init *= 1.5 / np.linalg.norm(init)
+
+    def cd(*p):
+        qai.IintStateVector(init.reshape([2] * n))
+        qai.RY(p[0], qubits=[0])
+        return qai.expval(qai.PauliZ(qubits=[1]))
+    c = qai.Circuit(cd, n, 0.3).compilecircuit(backend="pytorch")
+    c(torch.tensor([0.3], device=DEV))
+    assert abs(float(c.last_norm[0]) - 2.25) < 1e-5
+    strict = qai.Circuit(cd, n, 0.3).compilecircuit(backend="pytorch", norm_tol=1e-4)
+    with pytest.raises(EngineError, match="norm check failed"):
+        strict(torch.tensor([0.3], device=DEV))
+
+
+def test_more_than_32_z_string_outputs():
+    """40 Z-string expectation values (12 single Z_i and 28 products Z_i Z_j) of one circuit: reduced in
+    two passes of at most 32 masks, gradient seeds past the 32nd accumulated one pass each; values and
+    the gradient of a weighted sum against the oracle."""
+    n = 12
+    rng = np.random.default_rng(40)
+    spec = _random_spec(n, 50, rng)
+    pairs = [(i, j) for i in range(n) for j in range(i + 1, n)][:28]
+
+    def meas():
+        out = [qai.expval(qai.PauliZ(qubits=[q])) for q in range(n)]
+        out += [qai.expval([qai.PauliZ(qubits=[i]), qai.PauliZ(qubits=[j])]) for i, j in pairs]
+        return out
+    c, npar = _build(n, spec, meas)
+    assert len(c.executor().zmasks) == 40
+    vals = rng.uniform(-math.pi, math.pi, npar)
+    w = rng.normal(size=40)
+    t = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = c(t)
+    (out * torch.tensor(w, dtype=torch.float32, device=DEV)).sum().backward()
+    t64 = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    obs = [("expval", ("PauliZ", [q])) for q in range(n)]
+    obs += [("expval", [("PauliZ", [i]), ("PauliZ", [j])]) for i, j in pairs]
+    ref = O.execute(n, _oracle_gates(spec, list(t64)), obs)
+    (ref * torch.tensor(w)).sum().backward()
+    e_v = relerr(out.detach().cpu().numpy(), ref.detach().numpy())
+    e_g = relerr(t.grad.cpu().numpy(), t64.grad.numpy(), GFLOOR)
+    print(f"40 Z-string outputs: values relerr {e_v:.2e}, gradients relerr {e_g:.2e}", flush=True)
+    assert e_v < TOL and e_g < TOL
