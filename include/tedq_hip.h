@@ -85,6 +85,15 @@ int64_t tedq_sv_reduce_chunks(int n_qubits);
  * (pytorch_backend.py:399-459). */
 int tedq_sv_expval_z(const void* state, int64_t batch, int n_qubits, const uint64_t* dev_masks, int n_masks,
                      double* partial, double* out, void* stream);
+/* tedq_sv_expval_z without the partial buffer: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums go to a fixed 8 MiB device array inside the library (one per
+ * GPU, held until the process ends), so batch * tedq_sv_reduce_chunks(n) *
+ * n_masks * 8 must be <= 8 MiB (else -1 with a message, nothing launched).
+ * Allocates nothing.  Calls that overlap on one GPU share the array and race. */
+int tedq_sv_expval_z_noalloc(const void* state, int64_t batch, int n_qubits, const uint64_t* dev_masks,
+                             int n_masks, double* out, void* stream);
 /* lam = (sum_m dy[b][m] Z-string_m) psi; n_masks may be 0 (lam = 0).  dy: dev double [batch][n_masks]. */
 int tedq_sv_lambda_z(const void* psi, void* lam, int64_t batch, int n_qubits, const uint64_t* dev_masks,
                      int n_masks, const double* dy, void* stream);

@@ -6,6 +6,8 @@
 // Here all diagonal (Z-string) observables of a circuit are reduced in ONE
 // pass, accumulated in fp64 per thread and combined across workgroups with
 // Neumaier-compensated fp64 summation; the mask 0 observable is the norm.
+#include <mutex>
+
 #include "tedq_common.h"
 
 namespace tedq {
@@ -524,15 +526,19 @@ extern "C" int64_t tedq_sv_reduce_chunks(int n_qubits) {
   return n_qubits <= kChunkLog ? 1 : (1ll << (n_qubits - kChunkLog));
 }
 
-extern "C" int tedq_sv_expval_z(const void* state, int64_t batch, int n_qubits, const uint64_t* dev_masks,
-                                int n_masks, double* partial, double* out, void* stream) {
-  if (state == nullptr || dev_masks == nullptr || partial == nullptr || out == nullptr || batch <= 0 ||
-      n_qubits < 1 || n_qubits > 40 || n_masks < 1 || n_masks > kMaxZ || batch > 65535) {
-    set_error("tedq_sv_expval_z: bad arguments (batch=%lld n=%d masks=%d)", (long long)batch, n_qubits, n_masks);
-    return kBadArgument;
+static int expval_z_args_ok(const void* state, int64_t batch, int n_qubits, const uint64_t* dev_masks, int n_masks,
+                            const double* out, const char* what) {
+  if (state == nullptr || dev_masks == nullptr || out == nullptr || batch <= 0 || n_qubits < 1 || n_qubits > 40 ||
+      n_masks < 1 || n_masks > kMaxZ || batch > 65535) {
+    set_error("%s: bad arguments (batch=%lld n=%d masks=%d)", what, (long long)batch, n_qubits, n_masks);
+    return 0;
   }
+  return 1;
+}
+
+static int launch_expval_z(const void* state, int64_t batch, int n_qubits, const uint64_t* dev_masks, int n_masks,
+                           double* partial, double* out, hipStream_t st) {
   const int64_t nchunk = tedq_sv_reduce_chunks(n_qubits);
-  hipStream_t st = static_cast<hipStream_t>(stream);
   if (n_qubits >= kWhtChunkLog)
     hipLaunchKernelGGL(k_expval_z_wht, dim3((unsigned)nchunk, (unsigned)batch), dim3(kRedThreads), 0, st,
                        static_cast<const float2*>(state), n_qubits, dev_masks, n_masks, partial, nchunk);
@@ -544,6 +550,71 @@ extern "C" int tedq_sv_expval_z(const void* state, int64_t batch, int n_qubits, 
   hipLaunchKernelGGL(k_sum_partials<double>, dim3(nblocks(batch * n_masks, 128)), dim3(128), 0, st, partial, batch, nchunk,
                      n_masks, out);
   return check_launch("tedq_sv_expval_z(sum)");
+}
+
+extern "C" int tedq_sv_expval_z(const void* state, int64_t batch, int n_qubits, const uint64_t* dev_masks,
+                                int n_masks, double* partial, double* out, void* stream) {
+  if (!expval_z_args_ok(state, batch, n_qubits, dev_masks, n_masks, out, "tedq_sv_expval_z")) return kBadArgument;
+  if (partial == nullptr) {
+    set_error("tedq_sv_expval_z: bad arguments (null partial buffer)");
+    return kBadArgument;
+  }
+  return launch_expval_z(state, batch, n_qubits, dev_masks, n_masks, partial, out, static_cast<hipStream_t>(stream));
+}
+
+// ---------------------------------------------------------------------------
+// tedq_sv_expval_z without a caller buffer, for callers that cannot allocate.
+// The partial sums live in `scratch`, a file-scope device array: every GPU the
+// library runs on holds one copy of it until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in for
+// the symbol): the device address of each GPU's copy is looked up once per
+// device with hipGetSymbolAddress, with that device current, and kept in
+// g_scratch_dev.  The partial kernels write every entry the sum reads, so no
+// clearing is needed; calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocScratchBytes = 8ll << 20;
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+static double* g_scratch_dev[kMaxScratchDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_for_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxScratchDevices) {
+    set_error("tedq_sv_expval_z_noalloc: no usable current device (%s)", hipGetErrorString(e));
+    return kLaunchFailed;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("tedq_sv_expval_z_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return kLaunchFailed;
+    }
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return kOk;
+}
+
+extern "C" int tedq_sv_expval_z_noalloc(const void* state, int64_t batch, int n_qubits, const uint64_t* dev_masks,
+                                        int n_masks, double* out, void* stream) {
+  if (!expval_z_args_ok(state, batch, n_qubits, dev_masks, n_masks, out, "tedq_sv_expval_z_noalloc"))
+    return kBadArgument;
+  const int64_t need = batch * tedq_sv_reduce_chunks(n_qubits) * n_masks * (int64_t)sizeof(double);
+  if (need > kNoAllocScratchBytes) {
+    set_error("tedq_sv_expval_z_noalloc: needs %lld bytes of partial sums (batch=%lld n=%d masks=%d), "
+              "the device scratch holds %lld; use tedq_sv_expval_z with a caller buffer",
+              (long long)need, (long long)batch, n_qubits, n_masks, (long long)kNoAllocScratchBytes);
+    return kBadArgument;
+  }
+  double* partial = nullptr;
+  int rc = scratch_for_current_device(&partial);
+  if (rc) return rc;
+  return launch_expval_z(state, batch, n_qubits, dev_masks, n_masks, partial, out, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int tedq_sv_lambda_z(const void* psi, void* lam, int64_t batch, int n_qubits, const uint64_t* dev_masks,

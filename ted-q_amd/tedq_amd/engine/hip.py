@@ -29,6 +29,7 @@ _SIGNATURES = {
     "tedq_sv_init_copy": ([c_vp, c_vp, c_i64, c_int, c_vp], c_int),
     "tedq_sv_reduce_chunks": ([c_int], c_i64),
     "tedq_sv_expval_z": ([c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_vp], c_int),
+    "tedq_sv_expval_z_noalloc": ([c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp], c_int),
     "tedq_sv_lambda_z": ([c_vp, c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp], c_int),
     "tedq_sv_expval_factored": ([c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp], c_int),
     "tedq_sv_lambda_factored": ([c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_u64, c_vp, c_vp], c_int),

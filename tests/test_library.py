@@ -39,6 +39,18 @@ def test_library_rejects_bad_arguments_without_gpu_work():
 
 
 @pytest.mark.skipif(not os.path.exists(hip.library_path()), reason="library not built")
+def test_no_alloc_expval_checks_its_array_size_on_the_host():
+    """The no-allocation expectation entry point refuses inputs whose partial sums exceed its fixed 8 MiB
+    device array before touching the device (the pointers below are never dereferenced)."""
+    lib = hip.load()
+    dummy = ctypes.c_void_p(16)
+    rc = lib.tedq_sv_expval_z_noalloc(dummy, 160, 22, dummy, 32, dummy, None)
+    assert rc != 0 and b"device scratch holds 8388608" in lib.tedq_last_error()
+    rc = lib.tedq_sv_expval_z_noalloc(None, 4, 10, dummy, 2, dummy, None)
+    assert rc != 0 and b"bad arguments" in lib.tedq_last_error()
+
+
+@pytest.mark.skipif(not os.path.exists(hip.library_path()), reason="library not built")
 def test_contraction_entry_points_validate_on_the_host():
     """Launch planning and argument checks of the contraction entry points run on the host: workspace
     sizes of split and unsplit steps, the gate-backward eligibility rule, and records of a small-step
