@@ -39,6 +39,7 @@ from ..engine.sv import MeasSpec, check_device_flag
 from ..measurement import Expectation, Probability, State
 from .network import cancel_inverse_pairs
 from .path import external_path, path_sets, path_stats, search_path
+from .slicing import choose_slices, slice_network
 
 _BLOCK_CODE = {"RX": 14, "RY": 15, "RZ": 16, "Rot": 17, "PhaseShift": 18,
                "CRX": 14, "CRY": 15, "CRZ": 16, "ControlledPhaseShift": 18}
@@ -160,7 +161,7 @@ class _Step:
 class _Network:
     """One reduced network with its frozen path and index layouts."""
 
-    def __init__(self, tn, leaf_src, desc_rows, trials, external=None):
+    def __init__(self, tn, leaf_src, desc_rows, trials, external=None, slicing=None, sliced=None):
         # an externally found order applies to the network as built (no light-cone reduction)
         red = tn if external is not None else cancel_inverse_pairs(tn)
         inputs = [list(t) for t in red.input_indices]
@@ -172,9 +173,6 @@ class _Network:
                 external = [(0, 1)]
         self.measurement = tn.measurement
         self.output = list(tn.output_indices)
-        self.n_inputs = len(inputs)
-        self.sources = sources
-        self.leaves = [leaf_src(s) for s in sources]
         if external is not None:
             path = [tuple(p) for p in external]
             sets = path_sets(inputs, self.output, path)
@@ -183,6 +181,22 @@ class _Network:
                     "macs": sum(1 << len(sets[i] | sets[j]) for i, j in path), "largest_rank": largest}
         else:
             path, sets, info = search_path(inputs, self.output, trials)
+        # sliced indices: given with an external tree, or chosen on the frozen path from slicing options
+        sliced = list(sliced) if sliced else []
+        if not sliced and slicing:
+            sliced = choose_slices(path, sets, len(inputs), self.output, slicing.get("target_size"),
+                                   slicing.get("target_slices"))
+        self.sliced = sliced
+        self.sel_off = -1          # first constant slot of the selector vectors (set by the executor)
+        if sliced:
+            inputs, sources, path = slice_network(inputs, sources, self.output, path, sliced)
+            sets = path_sets(inputs, self.output, path)
+            largest, _ = path_stats(path, sets, len(inputs))
+            info = dict(info, sliced=list(sliced), slices=1 << len(sliced), largest_rank=largest,
+                        macs=sum(1 << len(sets[i] | sets[j]) for i, j in path) << len(sliced))
+        self.n_inputs = len(inputs)
+        self.sources = sources
+        self.leaves = [leaf_src(s) for s in sources]
         self.path_info = info
         self.path = path
         lay = [list(t) for t in inputs]
@@ -233,15 +247,16 @@ class _Network:
 
 
 def _external_paths(cc, paths):
-    """Per network: an SSA path from ``paths`` (SSA or linear, one per measurement), or from the
-    optimiser object the circuit was compiled with (the reference's ``use_cotengra`` / ``use_jdopttn``
-    calls, tedq/backends/compiled_circuit.py:340-393), or None (own search)."""
+    """Per network: (SSA path, sliced indices) from ``paths`` (SSA or linear, or path objects, one per
+    measurement), or from the optimiser object the circuit was compiled with (the reference's
+    ``use_cotengra`` / ``use_jdopttn`` calls, tedq/backends/compiled_circuit.py:340-393), or
+    (None, None) for the own search."""
     # pylint: disable=protected-access
     nets = cc._tensor_networks
     if paths is not None:
         if len(paths) != len(nets):
             raise ValueError(f"{len(paths)} contraction paths given for {len(nets)} networks")
-        return [external_path(p, tn.num_tensors) for p, tn in zip(paths, nets)]
+        return [(external_path(p, tn.num_tensors), sliced_indices(p)) for p, tn in zip(paths, nets)]
     hyper = cc._hyper_opt if isinstance(cc._hyper_opt, dict) else {}
     ctg, jd = cc._use_cotengra, cc._use_jdopttn
     out = []
@@ -258,8 +273,17 @@ def _external_paths(cc, paths):
             tree = jd(tn.input_indices, tn.size_dict, output=tn.output_indices, imbalance=0.2,
                       max_repeats=hyper.get("max_repeats", 128), search_parallel=hyper.get("search_parallel", True),
                       slicing_opts=hyper.get("slicing_opts", None))
-        out.append(None if tree is None else external_path(tree, tn.num_tensors))
+        out.append((None, None) if tree is None else (external_path(tree, tn.num_tensors), sliced_indices(tree)))
     return out
+
+
+def sliced_indices(tree):
+    """Index symbols an external contraction tree slices (cotengra's ``sliced_inds``, a dict or a
+    sequence), or None."""
+    got = getattr(tree, "sliced_inds", None)
+    if not got:
+        return None
+    return sorted(got.keys()) if isinstance(got, dict) else sorted(got)
 
 
 class TNExecutor:
@@ -270,7 +294,7 @@ class TNExecutor:
     ``specs`` and ``n_params``.
     """
 
-    def __init__(self, cc, trials=None, max_bytes=None, paths=None, gemm_form="4m"):
+    def __init__(self, cc, trials=None, max_bytes=None, paths=None, gemm_form="4m", slicing=None):
         from .path import SEARCH_TRIALS  # pylint: disable=import-outside-toplevel
         trials = SEARCH_TRIALS if trials is None else trials
         self.n = cc._num_qubits  # pylint: disable=protected-access
@@ -347,11 +371,26 @@ class TNExecutor:
         self.networks = []
         self.specs = []
         ext = _external_paths(cc, paths)
+        # own slicing: explicit options, else the reference's hyper_opt["slicing_opts"] (cotengra keys
+        # target_size = largest intermediate in elements, target_slices = least number of slices)
+        if slicing is None and isinstance(cc._hyper_opt, dict):  # pylint: disable=protected-access
+            slicing = cc._hyper_opt.get("slicing_opts")  # pylint: disable=protected-access
         for tn in cc._tensor_networks:  # pylint: disable=protected-access
             meas = tn.measurement
 
-            def leaf_src(s, meas=meas):
+            selectors = {}
+
+            def leaf_src(s, meas=meas, selectors=selectors):
                 kind = s[0]
+                if kind == "select":
+                    # selector vector of a sliced index (both caps of index j share one slot; the
+                    # slots of a network are contiguous and rewritten before each slice)
+                    if s[1] not in selectors:
+                        consts.append(np.array([1.0, 0.0], dtype=np.complex64))
+                        const_off.append(self._const_len)
+                        selectors[s[1]] = self._const_len
+                        self._const_len += 2
+                    return ("const", selectors[s[1]])
                 if kind == "gate":
                     return ("arena", self._gate_off[s[1]][0])
                 if kind == "adj":
@@ -368,7 +407,13 @@ class TNExecutor:
                 self._const_len += m.size
                 return ("const", o)
 
-            net = _Network(tn, leaf_src, desc_rows, trials, ext[len(self.networks)])
+            ext_path, ext_sliced = ext[len(self.networks)]
+            net = _Network(tn, leaf_src, desc_rows, trials, ext_path, slicing, ext_sliced)
+            if net.sliced:
+                offs = [selectors[j] for j in range(len(net.sliced))]
+                if offs != list(range(offs[0], offs[0] + 2 * len(offs), 2)):
+                    raise RuntimeError("tensor network: selector slots are not contiguous")
+                net.sel_off = offs[0]
             self.networks.append(net)
             if meas.return_type is Expectation:
                 self.specs.append(MeasSpec("fexp", ()))
@@ -394,6 +439,7 @@ class TNExecutor:
             main = desc_rows(ca if pa >= 0 else la, cb if pb >= 0 else lb, lc)
             self._gemm[row] = (pa, len(ca), pb, len(cb), main)
         self._consts = np.concatenate(consts).astype(np.complex64)
+        self._sel = {}
         self._descs = np.stack(descs).astype(np.int32) if descs else np.zeros((1, _DESC_WORDS), np.int32)
         self._dev = {}
 
@@ -528,8 +574,35 @@ class TNExecutor:
                  hdesc, ddesc, hip.ptr(ws), ws_bytes, hip.stream_handle(dev))
         return ws
 
+    def _set_selectors(self, net, a, dev):
+        """Write the selector vectors of slice ``a`` (bit j of ``a`` = value of sliced index j) into the
+        network's constant slots; stream-ordered after the previous slice's launches."""
+        key = (str(dev), id(net))
+        sel = self._sel.get(key)
+        if sel is None:
+            s = len(net.sliced)
+            vals = np.zeros((1 << s, s, 2), dtype=np.complex64)
+            for i in range(1 << s):
+                for j in range(s):
+                    vals[i, j, (i >> j) & 1] = 1.0
+            sel = self._sel[key] = torch.from_numpy(vals.reshape(1 << s, 2 * s)).to(dev)
+        consts = self._buffers(dev)["consts"]
+        consts[net.sel_off:net.sel_off + sel.shape[1]].copy_(sel[a])
+
     def _forward_network(self, net, leaves, B, dev, keep=None):
-        """Contract one network; ``keep(tid)`` says which intermediates to retain.  Returns (root, kept)."""
+        """Contract one network; ``keep(tid)`` says which intermediates to retain.  Returns (root, kept).
+        A sliced network (forward only) is contracted once per slice and the roots are added in slice
+        order."""
+        if net.sliced and keep is None:
+            total = None
+            for a in range(1 << len(net.sliced)):
+                self._set_selectors(net, a, dev)
+                root, _ = self._forward_once(net, leaves, B, dev)
+                total = root.clone() if total is None else total.add_(root)
+            return total, {}
+        return self._forward_once(net, leaves, B, dev, keep)
+
+    def _forward_once(self, net, leaves, B, dev, keep=None):
         buf = self._buffers(dev)
         vals = {}
         kept = {}
@@ -600,6 +673,19 @@ class TNExecutor:
         return [list(range(s, min(s + seg, len(net.steps)))) for s in range(0, len(net.steps), seg)]
 
     def _backward_network(self, net, leaves, dmats, root_env, grad, B, dev):
+        """Reverse contraction tree of one network; gradients are added into ``grad``.  Returns the
+        network's root.  A sliced network runs forward and backward once per slice: the parameter
+        gradients of the slices add up in ``grad`` and their roots are added in slice order."""
+        if not net.sliced:
+            return self._backward_once(net, leaves, dmats, root_env, grad, B, dev)
+        total = None
+        for a in range(1 << len(net.sliced)):
+            self._set_selectors(net, a, dev)
+            root = self._backward_once(net, leaves, dmats, root_env, grad, B, dev)
+            total = root.clone() if total is None else total.add_(root)
+        return total
+
+    def _backward_once(self, net, leaves, dmats, root_env, grad, B, dev):
         buf = self._buffers(dev)
         segs = self._segments(net, B, dev)
         seg_of = {}
@@ -613,7 +699,7 @@ class TNExecutor:
         else:
             def keep(t):
                 return seg_of.get(t, -1) != seg_of.get(net.steps[net.consumer[t]].c, -1)
-        root, stored = self._forward_network(net, leaves, B, dev, keep)
+        root, stored = self._forward_once(net, leaves, B, dev, keep)
         env_arena = torch.zeros((B, self.leaf_stride), dtype=torch.complex64, device=dev)
         envs = {net.root: root_env}
         for si in range(len(segs) - 1, -1, -1):

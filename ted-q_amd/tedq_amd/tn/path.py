@@ -300,7 +300,8 @@ def external_path(tree, n_inputs):
     Accepts what cotengra's ``HyperOptimizer.search`` and JDtensorPath return
     (objects with ``get_ssa_path()`` or ``get_path()``), opt_einsum-style
     objects with a ``path`` attribute (linear), or a plain list of pairs
-    (linear).  Sliced trees are contracted whole.
+    (linear).  The indices such a tree slices (``sliced_inds``) are read
+    separately (tn/executor.py, ``sliced_indices``) and sliced the same way.
     """
     if hasattr(tree, "get_ssa_path"):
         return [tuple(int(x) for x in st) for st in tree.get_ssa_path()]

@@ -394,3 +394,104 @@ def test_linear_paths_given_per_measurement():
     assert np.max(np.abs(grad[0].numpy() - g)) < 1e-4
     with pytest.raises(ValueError):
         TNExecutor(cc, paths=[[(0, 1)] * (n - 2)])
+
+
+def _lattice_oracle(L, layers, theta):
+    n = L * L
+    it = iter(theta)
+    gates = []
+    for _ in range(layers):
+        for q in range(n):
+            gates.append(("RX", [q], [next(it)]))
+            gates.append(("RZ", [q], [next(it)]))
+        for r in range(L):
+            for c in range(L - 1):
+                gates.append(("CZ", [r * L + c, r * L + c + 1], []))
+        for r in range(L - 1):
+            for c in range(L):
+                gates.append(("CZ", [r * L + c, (r + 1) * L + c], []))
+    return gates
+
+
+def test_sliced_contraction_equals_the_unsliced_one_and_the_oracle():
+    """slicing_opts (the reference's hyper_opt key): the engine slices bond indices on its frozen path
+    until every intermediate fits target_size, runs the sliced network once per slice (selector vectors
+    in the constant arena) and adds values and gradients; they equal the unsliced run and the oracle.
+    On the 3x3 lattice of 2 layers <Z_4> has rank-11 intermediates; a target of 2^6 elements slices
+    six indices (64 slices)."""
+    from tedq_amd import workloads as W
+    L, layers, obs = 3, 2, [4, 0]
+    n = L * L
+    full = install(W.lattice(L, layers, observe=obs, tn_mode=True).executor())
+    sl_ex = install(W.lattice(L, layers, observe=obs, tn_mode=True,
+                              hyper_opt={"slicing_opts": {"target_size": 2 ** 6}}).executor())
+    rep_full, rep_sl = full.path_report(), sl_ex.path_report()
+    assert len(sl_ex.networks[0].sliced) >= 4
+    assert max(r["largest_rank"] for r in rep_sl) <= 6 < max(r["largest_rank"] for r in rep_full)
+    rng = np.random.default_rng(11)
+    theta = rng.uniform(0, 2 * np.pi, size=(2, 2 * n * layers))
+    rows = torch.tensor(theta, dtype=torch.float32)
+    ups = [torch.full((2,), 0.5) for _ in obs]
+    v_f, g_f = full.value_and_grad(rows, ups)
+    v_s, g_s = sl_ex.value_and_grad(rows, ups)
+    assert np.max(np.abs(torch.stack(v_f).numpy() - torch.stack(v_s).numpy())) < 2e-6
+    assert np.max(np.abs(g_f.numpy() - g_s.numpy())) < 2e-6
+    # forward only (no backward): the sliced roots added per slice
+    run = sl_ex.run_forward(rows)
+    assert np.max(np.abs(torch.stack(sl_ex.measure(run)).numpy() - torch.stack(v_f).numpy())) < 2e-6
+    th = torch.tensor(theta[1], dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, _lattice_oracle(L, layers, th), [("expval", ("PauliZ", [q])) for q in obs])
+    (ref.sum() / 2).backward()
+    assert np.max(np.abs(np.array([float(v[1]) for v in v_s]) - ref.detach().numpy())) < 2e-5
+    assert np.max(np.abs(g_s[1].numpy() - th.grad.numpy())) < 1e-4
+
+
+def test_slices_of_an_external_tree_are_honoured():
+    """An external contraction tree that slices indices (cotengra's ``sliced_inds``) is contracted
+    along its path with those indices sliced, not whole."""
+
+    class SlicedTree:
+        def __init__(self, inputs, output):
+            self.ssa, sets = greedy_path(inputs, output)
+            counts = {}
+            for t in inputs:
+                for ix in t:
+                    counts[ix] = counts.get(ix, 0) + 1
+            widest = max(sets[len(inputs):], key=len)
+            self.sliced_inds = {ix: None for ix in sorted(widest) if counts.get(ix) == 2}
+
+        def get_ssa_path(self):
+            return self.ssa
+
+    cc = _small_expval_circuit(tn_mode=True)
+    tn = cc._tensor_networks[0]  # pylint: disable=protected-access
+    tree = SlicedTree(tn.input_indices, tn.output_indices)
+    assert tree.sliced_inds
+    ex = install(TNExecutor(cc, paths=[tree]))
+    assert ex.networks[0].sliced == sorted(tree.sliced_inds)
+    theta = np.array([0.7, -1.2, 0.4])
+    outs, grad = ex.value_and_grad(torch.tensor(theta[None], dtype=torch.float32), [torch.ones(1)])
+    v, g = _small_reference(theta)
+    assert abs(float(outs[0][0]) - v) < 2e-5
+    assert np.max(np.abs(grad[0].numpy() - g)) < 1e-4
+
+
+def test_choose_slices_reaches_its_targets():
+    from tedq_amd.tn.slicing import choose_slices
+    from tedq_amd import workloads as W
+    ex = W.disordered_chain(16, 3, 8, tn_mode=True).executor()
+    net = ex.networks[0]
+    inputs = [frozenset(t) for t in net.lay[:net.n_inputs]]
+    sets = path_sets_of(net)
+    for target in (2 ** 8, 2 ** 6):
+        s = choose_slices(net.path, sets, net.n_inputs, net.output, target_size=target)
+        assert max(len(t - set(s)) for t in sets[net.n_inputs:]) <= int(np.log2(target))
+    s = choose_slices(net.path, sets, net.n_inputs, net.output, target_slices=8)
+    assert len(s) == 3
+    assert not set(s) & set(net.output)
+    assert len(inputs) == net.n_inputs
+
+
+def path_sets_of(net):
+    from tedq_amd.tn.path import path_sets
+    return path_sets([list(t) for t in net.lay[:net.n_inputs]], net.output, net.path)
