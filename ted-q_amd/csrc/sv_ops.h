@@ -12,7 +12,19 @@ namespace tedq {
 enum OpCode : int {
   kOpU1 = 1, kOpD1 = 2, kOpCU1 = 3, kOpCX = 4, kOpCZ = 5,  // 6: unused
   kOpGrad1 = 7, kOpCGrad1 = 8, kOpCD1 = 9,
+  // real 2x2 (imaginary parts of the table entries are zero), optionally controlled
+  kOpR1 = 10,
+  // one gradient value of a one-parameter rotation whose derivative is dU U^dagger = i * (Pauli or
+  // projector) / const: words {11, bit, 0, cmask, 0, -1, kind | negate << 4, component}; the value is
+  // added to component `component` of the pending gradient group
+  kOpPauliGrad = 11,
+  // write the pending gradient group (8 values, reduced over the tile) to slot word[5] and clear it
+  kOpGradFlush = 12,
 };
+
+// kinds of kOpPauliGrad: the value equals 2 Re sum_ab A_ab C_ab for A = dU U^dagger of RX (X), RY (Y),
+// RZ (Z) and PhaseShift (P), with C_ab = sum over amplitude pairs of conj(lambda_a) psi_b
+enum PauliGradKind : int { kGradX = 0, kGradY = 1, kGradZ = 2, kGradP = 3 };
 
 constexpr int kHdrTileBits = 4;
 constexpr int kHdrHiOff = 36;
@@ -83,6 +95,47 @@ struct CZOp {  // -1 where bit B and all CM bits are 1
 #pragma unroll
     for (int i = 0; i < R; ++i)
       if ((i & (1 << B)) && (i & CM) == CM) v[i] = cneg(v[i]);
+  }
+};
+
+template <int R, int B, int CM>
+struct R1Op {  // real 2x2 on bit B where all CM bits are 1 (half the arithmetic of U1Op)
+  static __device__ __forceinline__ void run(float2 (&v)[R], float m0, float m1, float m2, float m3) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & (1 << B)) || (i & CM) != CM) continue;
+      const int j = i | (1 << B);
+      const float2 x = v[i], y = v[j];
+      v[i] = make_float2(fmaf(m0, x.x, m1 * y.x), fmaf(m0, x.y, m1 * y.y));
+      v[j] = make_float2(fmaf(m2, x.x, m3 * y.x), fmaf(m2, x.y, m3 * y.y));
+    }
+  }
+};
+
+// Gradient value of a one-parameter rotation (see PauliGradKind) over the pairs of bit B where all CM
+// bits are 1; NEG flips the sign (a daggered gate).  X: Im C01 + Im C10, Y: Re C10 - Re C01,
+// Z: Im C00 - Im C11, P: -2 Im C11.  Re conj(a) b = a.x b.x + a.y b.y, Im conj(a) b = a.x b.y - a.y b.x.
+template <int R, int B, int CM, int KIND, bool NEG>
+struct PauliGradOp {
+  static __device__ __forceinline__ void run(const float2 (&p)[R], const float2 (&l)[R], float& acc) {
+    const float s = NEG ? -1.f : 1.f;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if ((i & (1 << B)) || (i & CM) != CM) continue;
+      const int j = i | (1 << B);
+      if (KIND == kGradX) {
+        acc = fmaf(s * l[i].x, p[j].y, fmaf(-s * l[i].y, p[j].x, acc));
+        acc = fmaf(s * l[j].x, p[i].y, fmaf(-s * l[j].y, p[i].x, acc));
+      } else if (KIND == kGradY) {
+        acc = fmaf(s * l[j].x, p[i].x, fmaf(s * l[j].y, p[i].y, acc));
+        acc = fmaf(-s * l[i].x, p[j].x, fmaf(-s * l[i].y, p[j].y, acc));
+      } else if (KIND == kGradZ) {
+        acc = fmaf(s * l[i].x, p[i].y, fmaf(-s * l[i].y, p[i].x, acc));
+        acc = fmaf(-s * l[j].x, p[j].y, fmaf(s * l[j].y, p[j].x, acc));
+      } else {
+        acc = fmaf(-2.f * s * l[j].x, p[j].y, fmaf(2.f * s * l[j].y, p[j].x, acc));
+      }
+    }
   }
 };
 

@@ -18,7 +18,8 @@
 //   [36..100) 64-bit physical offset of register-row m in the load layout
 //   [100 + 48 s ...] stage s: [0] nops [1] op word offset [2] non-register
 //             local-bit mask [3] register mask [4..36) swizzled local offset of register m
-//   ops: 8 words {type | dagger<<8, b0, b1, cmask, mat_off, gslot, 0, 0}
+//   ops: 8 words {type | dagger<<8, b0, b1, cmask, mat_off, gslot, w6, w7} (w6/w7: kOpPauliGrad's
+//   kind and component, sv_ops.h)
 #include "sv_ops.h"
 
 namespace tedq {
@@ -64,6 +65,35 @@ __device__ __forceinline__ void grad_rt(const float2 (&p)[R], const float2 (&l)[
     acc[6] = fmaf(l[j].x, p[j].x, fmaf(l[j].y, p[j].y, acc[6]));
     acc[7] = fmaf(l[j].x, p[j].y, fmaf(-l[j].y, p[j].x, acc[7]));
   }
+}
+template <int R, int B>
+__device__ __forceinline__ void r1_rt(float2 (&v)[R], float m0, float m1, float m2, float m3, int cmask) {
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    if (i & (1 << B)) continue;
+    if ((i & cmask) != cmask) continue;
+    const int j = i | (1 << B);
+    const float2 x = v[i], y = v[j];
+    v[i] = make_float2(fmaf(m0, x.x, m1 * y.x), fmaf(m0, x.y, m1 * y.y));
+    v[j] = make_float2(fmaf(m2, x.x, m3 * y.x), fmaf(m2, x.y, m3 * y.y));
+  }
+}
+// gradient value of a one-parameter rotation (PauliGradOp in sv_ops.h) with a runtime kind and mask
+template <int R, int B>
+__device__ __forceinline__ float pauli_grad_rt(const float2 (&p)[R], const float2 (&l)[R], int cmask, int kind) {
+  float acc = 0.f;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    if (i & (1 << B)) continue;
+    if ((i & cmask) != cmask) continue;
+    const int j = i | (1 << B);
+    const float re_ij = fmaf(l[i].x, p[j].x, l[i].y * p[j].y), im_ij = fmaf(l[i].x, p[j].y, -l[i].y * p[j].x);
+    const float re_ji = fmaf(l[j].x, p[i].x, l[j].y * p[i].y), im_ji = fmaf(l[j].x, p[i].y, -l[j].y * p[i].x);
+    const float im_ii = fmaf(l[i].x, p[i].y, -l[i].y * p[i].x), im_jj = fmaf(l[j].x, p[j].y, -l[j].y * p[j].x);
+    acc += kind == kGradX ? im_ij + im_ji : kind == kGradY ? re_ji - re_ij : kind == kGradZ ? im_ii - im_jj
+                                                                                             : -2.f * im_jj;
+  }
+  return acc;
 }
 template <int R>
 __device__ __forceinline__ void diag_rt(float2 (&v)[R], int b, int cmask, float2 d0, float2 d1) {
@@ -128,6 +158,24 @@ __device__ __forceinline__ void op_u1(float2 (&v)[R], int b, int cmask, float2 m
 }
 
 template <int R>
+__device__ __forceinline__ void op_r1(float2 (&v)[R], int b, int cmask, float m0, float m1, float m2, float m3) {
+  if (by_bit<R, R1Op>(b, cmask, v, m0, m1, m2, m3)) return;
+#define CALL(B) r1_rt<R, B>(v, m0, m1, m2, m3, cmask)
+  TEDQ_BIT_SWITCH(b, CALL)
+#undef CALL
+}
+
+template <int R>
+__device__ __forceinline__ float op_pauli_grad(const float2 (&p)[R], const float2 (&l)[R], int b, int cmask,
+                                               int kind) {
+  float v = 0.f;
+#define CALL(B) v = pauli_grad_rt<R, B>(p, l, cmask, kind)
+  TEDQ_BIT_SWITCH(b, CALL)
+#undef CALL
+  return v;
+}
+
+template <int R>
 __device__ __forceinline__ void op_cx(float2 (&v)[R], int b, int cmask) {
   if (by_bit<R, CXOp>(b, cmask, v)) return;
 #define CALL(B) cx_rt<R, B>(v, cmask)
@@ -179,6 +227,14 @@ __device__ __forceinline__ void apply_op(float2 (&v)[R], float2 (&w)[R], const i
       if (TWO) op_u1<R>(w, op[1], op[3], m0, m1, m2, m3);
       break;
     }
+    case kOpR1: {
+      const float2* m = mb + op[4];
+      const float m0 = m[0].x, m3 = m[3].x;
+      const float m1 = dag ? m[2].x : m[1].x, m2 = dag ? m[1].x : m[2].x;
+      op_r1<R>(v, op[1], op[3], m0, m1, m2, m3);
+      if (TWO) op_r1<R>(w, op[1], op[3], m0, m1, m2, m3);
+      break;
+    }
     case kOpD1:
     case kOpCD1: {
       const float2* m = mb + op[4];
@@ -226,10 +282,10 @@ struct TileCtx {
 // each element costs one XOR.
 template <int R, bool MULTI>
 __device__ __forceinline__ void load_tiles(const float2* __restrict__ src, float2* lds, const int* __restrict__ prog,
-                                           const PassArgs& a, int tid, int T, const TileCtx& c) {
+                                           const PassArgs& a, int tid, int T, const TileCtx& c, int64_t blk) {
   const int sw = swz(tid);
   if (MULTI) {
-    const int64_t wg_base = (int64_t)blockIdx.x * T * R;
+    const int64_t wg_base = blk * T * R;
     const int64_t left = (a.batch << a.n) - wg_base;
     const int64_t cnt = left < (int64_t)T * R ? left : (int64_t)T * R;
     const __amdgpu_buffer_rsrc_t r = make_rsrc(src + wg_base, (uint32_t)(cnt * 8));
@@ -251,10 +307,10 @@ __device__ __forceinline__ void load_tiles(const float2* __restrict__ src, float
 
 template <int R, bool MULTI>
 __device__ __forceinline__ void store_tiles(float2* __restrict__ dst, const float2* lds, const int* __restrict__ prog,
-                                            const PassArgs& a, int tid, int T, const TileCtx& c) {
+                                            const PassArgs& a, int tid, int T, const TileCtx& c, int64_t blk) {
   const int sw = swz(tid);
   if (MULTI) {
-    const int64_t wg_base = (int64_t)blockIdx.x * T * R;
+    const int64_t wg_base = blk * T * R;
     const int64_t left = (a.batch << a.n) - wg_base;
     const int64_t cnt = left < (int64_t)T * R ? left : (int64_t)T * R;
     const __amdgpu_buffer_rsrc_t r = make_rsrc(dst + wg_base, (uint32_t)(cnt * 8));
@@ -274,17 +330,18 @@ __device__ __forceinline__ void store_tiles(float2* __restrict__ dst, const floa
   }
 }
 
-__device__ __forceinline__ TileCtx make_ctx(const PassArgs& a, const int* __restrict__ prog, int tid, bool multi) {
+__device__ __forceinline__ TileCtx make_ctx(const PassArgs& a, const int* __restrict__ prog, int tid, bool multi,
+                                            int64_t blk) {
   TileCtx c;
   if (multi) {
-    const int64_t tile = ((int64_t)blockIdx.x << a.gT) + (tid >> a.tT);
+    const int64_t tile = (blk << a.gT) + (tid >> a.tT);
     c.circuit = tile;  // whole-circuit tiles
     c.valid = tile < a.batch;
     c.gbase = 0;
     c.lo = 0;
   } else {
     c.valid = true;
-    tile_origin(a, blockIdx.x, &c.circuit, &c.gbase);
+    tile_origin(a, blk, &c.circuit, &c.gbase);
     uint64_t lo = 0;
     for (int i = 0; i < a.tT; ++i)
       if ((tid >> i) & 1) lo |= 1ull << prog[kHdrTileBits + i];
@@ -296,15 +353,16 @@ __device__ __forceinline__ TileCtx make_ctx(const PassArgs& a, const int* __rest
 // ---------------------------------------------------------------------------
 // forward pass
 // ---------------------------------------------------------------------------
+// Workgroup `blk` of a forward pass (a tile, or G whole circuits when MULTI).
 template <int R, bool MULTI>
-__global__ void __launch_bounds__(512) k_pass_fwd(float2* __restrict__ state, const int* __restrict__ prog,
-                                                  const float2* __restrict__ mats, PassArgs a) {
+__device__ __forceinline__ void pass_fwd_body(float2* __restrict__ state, const int* __restrict__ prog,
+                                              const float2* __restrict__ mats, const PassArgs& a, int64_t blk) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x;
   const int T = blockDim.x;
-  const TileCtx c = make_ctx(a, prog, tid, MULTI);
+  const TileCtx c = make_ctx(a, prog, tid, MULTI, blk);
   const float2* mb = mats + (c.valid ? c.circuit : 0) * a.mat_stride;
-  load_tiles<R, MULTI>(state, lds, prog, a, tid, T, c);
+  load_tiles<R, MULTI>(state, lds, prog, a, tid, T, c, blk);
   __syncthreads();
   const int nst = prog[2];
   for (int s = 0; s < nst; ++s) {
@@ -320,7 +378,13 @@ __global__ void __launch_bounds__(512) k_pass_fwd(float2* __restrict__ state, co
     for (int m = 0; m < R; ++m) lds[sb ^ st[4 + m]] = v[m];
     __syncthreads();
   }
-  store_tiles<R, MULTI>(state, lds, prog, a, tid, T, c);
+  store_tiles<R, MULTI>(state, lds, prog, a, tid, T, c, blk);
+}
+
+template <int R, bool MULTI>
+__global__ void __launch_bounds__(512) k_pass_fwd(float2* __restrict__ state, const int* __restrict__ prog,
+                                                  const float2* __restrict__ mats, PassArgs a) {
+  pass_fwd_body<R, MULTI>(state, prog, mats, a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -328,19 +392,19 @@ __global__ void __launch_bounds__(512) k_pass_fwd(float2* __restrict__ state, co
 // lambda; gradient ops reduce their 8 bilinears over the threads of a tile.
 // ---------------------------------------------------------------------------
 template <int R, bool MULTI>
-__global__ void __launch_bounds__(512) k_pass_adj(float2* __restrict__ psi, float2* __restrict__ lam,
-                                                  const int* __restrict__ prog, const float2* __restrict__ mats,
-                                                  float* __restrict__ partials, PassArgs a) {
+__device__ __forceinline__ void pass_adj_body(float2* __restrict__ psi, float2* __restrict__ lam,
+                                              const int* __restrict__ prog, const float2* __restrict__ mats,
+                                              float* __restrict__ partials, const PassArgs& a, int64_t blk) {
   extern __shared__ float2 lds[];
   float2* ldp = lds;
   const int T = blockDim.x;
   float2* ldl = lds + T * R;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const TileCtx c = make_ctx(a, prog, tid, MULTI);
+  const TileCtx c = make_ctx(a, prog, tid, MULTI, blk);
   const float2* mb = mats + (c.valid ? c.circuit : 0) * a.mat_stride;
-  load_tiles<R, MULTI>(psi, ldp, prog, a, tid, T, c);
-  load_tiles<R, MULTI>(lam, ldl, prog, a, tid, T, c);
+  load_tiles<R, MULTI>(psi, ldp, prog, a, tid, T, c, blk);
+  load_tiles<R, MULTI>(lam, ldl, prog, a, tid, T, c, blk);
   __syncthreads();
 
   // partial rows: W waves per tile when a tile spans several waves
@@ -348,12 +412,14 @@ __global__ void __launch_bounds__(512) k_pass_adj(float2* __restrict__ psi, floa
   const int seg = tpt < 64 ? tpt : 64;       // lanes reduced together
   const int W = tpt >> 6 > 0 ? tpt >> 6 : 1; // partial rows per tile
   const int tl = tid & (tpt - 1);
-  int64_t tile_in_circuit = MULTI ? 0 : ((int64_t)blockIdx.x & ((1ll << (a.n - a.k)) - 1));
+  int64_t tile_in_circuit = MULTI ? 0 : (blk & ((1ll << (a.n - a.k)) - 1));
   const int64_t prow = tile_in_circuit * W + (tl >> 6);
   const bool seg_writer = c.valid && ((tid & (seg - 1)) == 0);
   const bool wave_writer = (lane & 7) == 0;
   float* const pbase = partials + (c.circuit * a.n_gslots * a.np + prow) * 8;
 
+  // pending group of one-parameter gradient values (kOpPauliGrad), written by kOpGradFlush
+  float sacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   const int nst = prog[2];
   for (int s = 0; s < nst; ++s) {
     const int* st = prog + kHdrStages + kStageWords * s;
@@ -368,9 +434,24 @@ __global__ void __launch_bounds__(512) k_pass_adj(float2* __restrict__ psi, floa
     const int* op = prog + st[1];
     for (int o = 0; o < nops; ++o, op += 8) {
       const int type = op[0] & 0xff;
-      if (type == kOpGrad1 || type == kOpCGrad1) {
+      if (type == kOpPauliGrad) {
+        float v = op_pauli_grad<R>(vp, vl, op[1], op[3], op[6] & 15);
+        if (op[6] >> 4) v = -v;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (q == op[7]) sacc[q] += v;
+      } else if (type == kOpGrad1 || type == kOpCGrad1 || type == kOpGradFlush) {
+        const bool flush = type == kOpGradFlush;
         float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        op_grad<R>(vp, vl, op[1], op[3], acc);
+        if (flush) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            acc[q] = sacc[q];
+            sacc[q] = 0.f;
+          }
+        } else {
+          op_grad<R>(vp, vl, op[1], op[3], acc);
+        }
         float* dst = pbase + (int64_t)op[5] * a.np * 8;
         if (seg == 64) {
           const float tot = reduce8_wave(acc, lane);
@@ -396,8 +477,80 @@ __global__ void __launch_bounds__(512) k_pass_adj(float2* __restrict__ psi, floa
     }
     __syncthreads();
   }
-  store_tiles<R, MULTI>(psi, ldp, prog, a, tid, T, c);
-  store_tiles<R, MULTI>(lam, ldl, prog, a, tid, T, c);
+  store_tiles<R, MULTI>(psi, ldp, prog, a, tid, T, c, blk);
+  store_tiles<R, MULTI>(lam, ldl, prog, a, tid, T, c, blk);
+}
+
+template <int R, bool MULTI>
+__global__ void __launch_bounds__(512) k_pass_adj(float2* __restrict__ psi, float2* __restrict__ lam,
+                                                  const int* __restrict__ prog, const float2* __restrict__ mats,
+                                                  float* __restrict__ partials, PassArgs a) {
+  pass_adj_body<R, MULTI>(psi, lam, prog, mats, partials, a, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// ragged passes: circuits of different structures and sizes in one launch
+// (a serving batch).  Every circuit has its own state, program, matrices and
+// partial rows; all share the tile bits k and register bits r, so one grid of
+// 2^(k-r)-thread workgroups covers the tiles of all of them.  Workgroup b
+// belongs to the first record whose tile_end exceeds b.
+// ---------------------------------------------------------------------------
+struct RaggedRec {
+  float2* psi;           // the circuit's state [2^n]
+  float2* lam;           // adjoint: its lambda [2^n]
+  const int* prog;       // its program for this pass (forward or adjoint)
+  const float2* mats;    // its slot matrices (one table row)
+  float* partials;       // adjoint: its partial rows [n_gslots][2^(n-k) * W][8]
+  uint64_t rest_mask;    // physical bits of the circuit that are not tile bits of this pass
+  int64_t tile_end;      // tiles of this and all earlier records
+  int n;
+  int n_gslots;
+};
+static_assert(sizeof(RaggedRec) == 64, "ragged record layout is part of the C ABI");
+
+__device__ __forceinline__ int ragged_record(const RaggedRec* __restrict__ recs, int nrec, int64_t blk) {
+  int lo = 0, hi = nrec - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (recs[mid].tile_end > blk) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ PassArgs ragged_args(const RaggedRec& rc, int k, int tT) {
+  PassArgs a;
+  a.mat_stride = 0;
+  a.batch = 1;
+  a.total_tiles = 1ll << (rc.n - k);
+  a.rest_mask = rc.rest_mask;
+  a.n = rc.n;
+  a.k = k;
+  a.tT = tT;
+  a.gT = 0;
+  const int W = tT > 6 ? 1 << (tT - 6) : 1;
+  a.np = (1ll << (rc.n - k)) * W;
+  a.n_gslots = rc.n_gslots;
+  return a;
+}
+
+template <int R>
+__global__ void __launch_bounds__(512) k_pass_fwd_ragged(const RaggedRec* __restrict__ recs, int nrec, int k,
+                                                         int tT) {
+  const int c = ragged_record(recs, nrec, blockIdx.x);
+  const RaggedRec rc = recs[c];
+  const int64_t first = c ? recs[c - 1].tile_end : 0;
+  pass_fwd_body<R, false>(rc.psi, rc.prog, rc.mats, ragged_args(rc, k, tT), (int64_t)blockIdx.x - first);
+}
+
+template <int R>
+__global__ void __launch_bounds__(512) k_pass_adj_ragged(const RaggedRec* __restrict__ recs, int nrec, int k,
+                                                         int tT) {
+  const int c = ragged_record(recs, nrec, blockIdx.x);
+  const RaggedRec rc = recs[c];
+  const int64_t first = c ? recs[c - 1].tile_end : 0;
+  pass_adj_body<R, false>(rc.psi, rc.lam, rc.prog, rc.mats, rc.partials, ragged_args(rc, k, tT),
+                          (int64_t)blockIdx.x - first);
 }
 
 // gfx950 provides 160 KiB of LDS per workgroup; above 64 KiB the kernel must opt in once.
@@ -501,9 +654,52 @@ static int run_pass(bool adjoint, void* psi, void* lam, int64_t batch, int n, in
   }
 }
 
+template <int R>
+static int launch_ragged(bool adjoint, const RaggedRec* recs, int nrec, int64_t tiles, int k, int r,
+                         hipStream_t st) {
+  const int tT = k - r;
+  const int T = 1 << tT;
+  const size_t lds = (size_t)T * R * sizeof(float2) * (adjoint ? 2 : 1);
+  bool ok;
+  if (adjoint) {
+    ok = allow_lds(k_pass_adj_ragged<R>, lds);
+    if (ok) hipLaunchKernelGGL(k_pass_adj_ragged<R>, dim3((unsigned)tiles), dim3(T), lds, st, recs, nrec, k, tT);
+  } else {
+    ok = allow_lds(k_pass_fwd_ragged<R>, lds);
+    if (ok) hipLaunchKernelGGL(k_pass_fwd_ragged<R>, dim3((unsigned)tiles), dim3(T), lds, st, recs, nrec, k, tT);
+  }
+  if (!ok) {
+    set_error("ragged pass: could not enable %zu bytes of LDS", lds);
+    return kLaunchFailed;
+  }
+  return check_launch("tedq_sv_pass_ragged");
+}
+
 }  // namespace tedq
 
 using namespace tedq;
+
+extern "C" int tedq_sv_pass_ragged(int adjoint, const void* dev_recs, int n_recs, int64_t total_tiles, int k,
+                                   int r, void* stream) {
+  if (dev_recs == nullptr || n_recs < 1 || total_tiles < n_recs || total_tiles > 0x7fffffffll || k < 1 ||
+      k > 20 || r < 1 || r > 5 || r > k || k - r > 9 || (1ll << (k - r)) * (1ll << r) * 8 * (adjoint ? 2 : 1) >
+      (int64_t)kMaxLds) {
+    set_error("tedq_sv_pass_ragged: bad arguments (records=%d tiles=%lld k=%d r=%d)", n_recs,
+              (long long)total_tiles, k, r);
+    return kBadArgument;
+  }
+  const RaggedRec* recs = static_cast<const RaggedRec*>(dev_recs);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool adj = adjoint != 0;
+  switch (r) {
+    case 1: return launch_ragged<2>(adj, recs, n_recs, total_tiles, k, r, st);
+    case 2: return launch_ragged<4>(adj, recs, n_recs, total_tiles, k, r, st);
+    case 3: return launch_ragged<8>(adj, recs, n_recs, total_tiles, k, r, st);
+    case 4: return launch_ragged<16>(adj, recs, n_recs, total_tiles, k, r, st);
+    case 5: return launch_ragged<32>(adj, recs, n_recs, total_tiles, k, r, st);
+    default: return kBadArgument;
+  }
+}
 
 extern "C" int tedq_sv_apply_pass(void* state, int64_t batch, int n_qubits, int tile_bits, int reg_bits,
                                   int threads_log2, const int* dev_prog, uint64_t tile_mask, const void* mats,

@@ -23,8 +23,11 @@ enum GateCode : int {
   kRX = 14, kRY = 15, kRZ = 16, kRot = 17, kPhaseShift = 18, kUnitary = 23,
 };
 
+// scalar: 0 for a slot whose gradients use the full bilinears C_ij of its gslot; c + 1 when the slot's
+// one trainable parameter gets its gradient value directly in component c of gslot (kOpPauliGrad): then
+// A is the constant that picks that component out of 2 Re sum A_ij C_ij.
 struct SlotRec {
-  int kind, ncomp, comp_start, mat_off, gslot, pad0, pad1, pad2;
+  int kind, ncomp, comp_start, mat_off, gslot, scalar, pad1, pad2;
 };
 struct CompRec {
   int code, dagger, src0, src1, src2, const_off, pad0, pad1;
@@ -186,6 +189,22 @@ __global__ void k_build_tables(const PT* __restrict__ params, int64_t batch, int
   for (int i = 0; i < 4; ++i) out[i] = make_float2((float)S.a[i].x, (float)S.a[i].y);
   if (!ok) atomicOr(err, 1);
   if (amats == nullptr || sr.gslot < 0) return;
+  if (sr.scalar > 0) {
+    // component c = Re (even) or Im (odd) of C_{c/2}: A_{c/2} = 1/2 or -i/2, the other entries 0
+    const int c = sr.scalar - 1;
+    for (int k = 0; k < sr.ncomp; ++k) {
+      const CompRec cr = comps[sr.comp_start + k];
+      const int srcs[3] = {cr.src0, cr.src1, cr.src2};
+      for (int w = 0; w < 3; ++w) {
+        if (srcs[w] == INT32_MIN || srcs[w] < 0) continue;
+        double2* dst = amats + (b * (int64_t)n_params + srcs[w]) * 4;
+        for (int i = 0; i < 4; ++i)
+          dst[i] = i == (c >> 1) ? ((c & 1) ? make_double2(0.0, -0.5) : make_double2(0.5, 0.0))
+                                 : make_double2(0.0, 0.0);
+      }
+    }
+    return;
+  }
   const M2 Sd = m2_dag(S);
   M2 prev = m2_eye();  // S_{i-1}
   for (int c = 0; c < sr.ncomp; ++c) {

@@ -21,7 +21,7 @@ import os
 import numpy as np
 
 from . import hip
-from .planner import OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1
+from .planner import OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1, OP_R1, OP_PGRAD, OP_GFLUSH
 from .program import HDR_TILE_BITS, HDR_HI_OFF, HDR_STAGES, STAGE_WORDS, OP_WORDS, lds_swizzle
 
 _CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))), "csrc")
@@ -63,12 +63,50 @@ def _diag_args(dag, off):
     return f"mb[{off}], mb[{off + 3}]"
 
 
+def _reduce_write_lines(var, seg):
+    """Sum the 8 values ``var`` over the tile's lanes and write them to ``dst`` (one partial row)."""
+    out = []
+    if seg == 64:
+        out.append(f"  const float tot = reduce8_wave({var}, lane);")
+        out.append("  if ((lane & 7) == 0) dst[(lane >> 3) & 7] = tot;")
+    else:
+        out.append("#pragma unroll")
+        out.append("  for (int q = 0; q < 8; ++q)")
+        out.append(f"    for (int o = 1; o < {seg}; o <<= 1) {var}[q] += __shfl_xor({var}[q], o, 64);")
+        out.append(f"  if ((tid & {seg - 1}) == 0) {{")
+        out.append("    float4* d4 = reinterpret_cast<float4*>(dst);")
+        out.append(f"    d4[0] = make_float4({var}[0], {var}[1], {var}[2], {var}[3]);")
+        out.append(f"    d4[1] = make_float4({var}[4], {var}[5], {var}[6], {var}[7]);")
+        out.append("  }")
+    return out
+
+
 def _op_lines(w, R, regs, adjoint, seg):
     """Source lines of one op (8 program words) applied to the register arrays ``regs``."""
     t, dag = int(w[0]) & 0xFF, (int(w[0]) >> 8) & 1
     b, cm, off, gslot = int(w[1]), int(w[3]), int(w[4]), int(w[5])
     out = []
-    if t in (OP_U1, OP_CU1):
+    if t == OP_R1:
+        m = [f"mb[{off + i}].x" for i in range(4)]
+        if dag:
+            m[1], m[2] = m[2], m[1]
+        for v in regs:
+            out.append(f"R1Op<{R}, {b}, {cm}>::run({v}, {', '.join(m)});")
+    elif t == OP_PGRAD:
+        if not adjoint:
+            raise ValueError("gradient op in a forward program")
+        kind, neg, comp = int(w[6]) & 15, int(w[6]) >> 4, int(w[7])
+        out.append(f"PauliGradOp<{R}, {b}, {cm}, {kind}, {'true' if neg else 'false'}>::run(vp, vl, sacc[{comp}]);")
+    elif t == OP_GFLUSH:
+        if not adjoint:
+            raise ValueError("gradient op in a forward program")
+        out.append("{")
+        out.append(f"  float* dst = pbase + (int64_t){gslot} * a.np * 8;")
+        out.extend(_reduce_write_lines("sacc", seg))
+        out.append("#pragma unroll")
+        out.append("  for (int q = 0; q < 8; ++q) sacc[q] = 0.f;")
+        out.append("}")
+    elif t in (OP_U1, OP_CU1):
         out.append("{")
         for v in regs:
             out.append(f"  U1Op<{R}, {b}, {cm}>::run({v}, {_u1_args(dag, off)});")
@@ -91,18 +129,7 @@ def _op_lines(w, R, regs, adjoint, seg):
         out.append("  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};")
         out.append(f"  GradOp<{R}, {b}, {cm}>::run(vp, vl, acc);")
         out.append(f"  float* dst = pbase + (int64_t){gslot} * a.np * 8;")
-        if seg == 64:
-            out.append("  const float tot = reduce8_wave(acc, lane);")
-            out.append("  if ((lane & 7) == 0) dst[(lane >> 3) & 7] = tot;")
-        else:
-            out.append("#pragma unroll")
-            out.append("  for (int q = 0; q < 8; ++q)")
-            out.append(f"    for (int o = 1; o < {seg}; o <<= 1) acc[q] += __shfl_xor(acc[q], o, 64);")
-            out.append(f"  if ((tid & {seg - 1}) == 0) {{")
-            out.append("    float4* d4 = reinterpret_cast<float4*>(dst);")
-            out.append("    d4[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);")
-            out.append("    d4[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);")
-            out.append("  }")
+        out.extend(_reduce_write_lines("acc", seg))
         out.append("}")
     else:
         raise ValueError(f"op type {t} has no generated form")
@@ -136,7 +163,8 @@ def pass_kernel_source(name, words, adjoint):
               "  const int lane = tid & 63;",
               f"  const int64_t prow = ((int64_t)blockIdx.x & ((1ll << (a.n - a.k)) - 1)) * {W} + (tid >> 6);",
               "  float* const pbase = partials + (circ * a.n_gslots * a.np + prow) * 8;",
-              "  (void)lane;"]
+              "  float sacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};",
+              "  (void)lane; (void)sacc;"]
     for m in range(R):
         L.append(f"  lds[sw ^ {lds_swizzle(m * T)}] = bload(rp, lo * 8u, {int(hi[m]) * 8}u);")
         if adjoint:

@@ -24,9 +24,14 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-# Op codes shared with csrc/sv_passes.hip (kOp* constants there).
+# Op codes shared with csrc/sv_ops.h (kOp* constants there).
 OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_U2, OP_GRAD1, OP_CGRAD1, OP_CD1 = 1, 2, 3, 4, 5, 6, 7, 8, 9
+OP_R1, OP_PGRAD, OP_GFLUSH = 10, 11, 12
 OP_DAGGER = 1 << 8
+
+# kinds of OP_PGRAD (PauliGradKind in csrc/sv_ops.h) by gate code: RX, RY, RZ, PhaseShift blocks
+PAULI_KIND = {14: 0, 15: 1, 16: 2, 18: 3}
+GROUP_SIZE = 8                                  # gradient values per OP_PGRAD group (one partial row)
 
 # Slot kinds shared with csrc/sv_tables.hip.
 SLOT_1Q, SLOT_2Q = 0, 1
@@ -35,6 +40,7 @@ SLOT_1Q, SLOT_2Q = 0, 1
 # row (>= 0) or a constant in the fp64 constant pool (encoded -1 - index).
 
 _DIAGONAL_CODES = {0, 4, 5, 6, 16, 18}          # I, Z, S, T, RZ, PhaseShift
+_REAL_CODES = {0, 1, 2, 4, 15}                  # I, H, X, Z, RY
 _ONEQ_CODES = {0, 1, 2, 3, 4, 5, 6, 7, 14, 15, 16, 17, 18, 23}
 _PARAM_COUNT = {14: 1, 15: 1, 16: 1, 17: 3, 18: 1}
 
@@ -55,6 +61,7 @@ class Slot:
     comps: list
     mat_off: int = 0       # complex64 offset inside one batch row of the table
     gslot: int = -1        # gradient slot index or -1
+    scomp: int = -1        # component inside gslot of a single-value gradient (OP_PGRAD), else -1
 
     @property
     def dim(self):
@@ -67,6 +74,21 @@ class Slot:
     @property
     def diagonal(self):
         return self.kind == SLOT_1Q and all(c.code in _DIAGONAL_CODES for c in self.comps)
+
+    @property
+    def real(self):
+        return self.kind == SLOT_1Q and all(c.code in _REAL_CODES and c.const_mat < 0 for c in self.comps)
+
+    @property
+    def pauli(self):
+        """(kind, negate) when the slot is one trainable rotation whose gradient is a single value
+        (RX, RY, RZ, PhaseShift and their controlled blocks), else None."""
+        if self.kind != SLOT_1Q or len(self.comps) != 1:
+            return None
+        c = self.comps[0]
+        if c.code not in PAULI_KIND or c.const_mat >= 0 or len(c.srcs) != 1 or c.srcs[0] < 0:
+            return None
+        return PAULI_KIND[c.code], bool(c.dagger)
 
 
 @dataclass
@@ -214,8 +236,14 @@ def _schedule(units, order, max_bits, seed_bits, allowed_bits=None):
     return done, chosen
 
 
-def build_plan(n, units, n_params, const_pool, k_max=12, r=4, coalesce_bits=3):
-    """Partition ``units`` (program order) into passes and register stages."""
+def build_plan(n, units, n_params, const_pool, k_max=12, r=4, coalesce_bits=3, specialize=False):
+    """Partition ``units`` (program order) into passes and register stages.
+
+    With ``specialize`` (the adjoint's plan) a trainable single-qubit gate is never fused with another
+    gate, so that a rotation keeps its own slot: real gates then run as real 2x2 ops and the gradient
+    of RX / RY / RZ / PhaseShift (and controlled forms) is one value per parameter (OP_PGRAD, 2-4
+    multiply-adds per amplitude pair) instead of the slot's 8 bilinears (16 per pair), collected in
+    groups of 8 per register stage."""
     k = min(n, k_max)
     widest = max((len(u.bits) for u in units), default=1)
     if widest > k:
@@ -240,21 +268,42 @@ def build_plan(n, units, n_params, const_pool, k_max=12, r=4, coalesce_bits=3):
         tile_bits = sorted(chosen)
         done_set = set(done)
         todo = [u for u in todo if u not in done_set]
-        passes.append(_build_pass(units, done, tile_bits, r, slots))
+        passes.append(_build_pass(units, done, tile_bits, r, slots, specialize))
     mat_off = 0
-    gslot = 0
     for s in slots:
         s.mat_off = mat_off
         mat_off += s.dim * s.dim
-        if s.trainable:
-            s.gslot = gslot
-            gslot += 1
+    gslot = 0
+    if specialize:
+        # single-value gradients share partial rows: groups of 8 per register stage, numbered in the
+        # adjoint's order (stage ops reversed)
+        for p in passes:
+            for st in p.stages:
+                group, used = -1, GROUP_SIZE
+                for op in reversed(st.ops):
+                    si = op[4]
+                    if si < 0 or not slots[si].trainable:
+                        continue
+                    if slots[si].pauli is not None:
+                        if used == GROUP_SIZE:
+                            group, used = gslot, 0
+                            gslot += 1
+                        slots[si].gslot, slots[si].scomp = group, used
+                        used += 1
+                    else:
+                        slots[si].gslot = gslot
+                        gslot += 1
+    else:
+        for s in slots:
+            if s.trainable:
+                s.gslot = gslot
+                gslot += 1
     return Plan(n=n, k=k, r=r, passes=passes, slots=slots,
                 const_pool=np.asarray(const_pool, dtype=np.complex128),
                 n_params=n_params, n_gslots=gslot, mat_stride=max(mat_off, 1))
 
 
-def _build_pass(units, pass_units, tile_bits, r, slots):
+def _build_pass(units, pass_units, tile_bits, r, slots, specialize=False):
     local = {b: i for i, b in enumerate(tile_bits)}
     allowed = set(tile_bits)
     p = Pass(tile_bits=tile_bits)
@@ -276,7 +325,9 @@ def _build_pass(units, pass_units, tile_bits, r, slots):
             if u.kind == "u1":
                 b = u.bits[0]
                 prev = last_on_bit.get(b)
-                if prev is not None and stage.ops[prev][0] == "u1":
+                if prev is not None and stage.ops[prev][0] == "u1" and (
+                        not specialize or (not slots[stage.ops[prev][4]].trainable
+                                           and not any(x >= 0 for x in u.comp.srcs))):
                     slots[stage.ops[prev][4]].comps.append(u.comp)
                     continue
                 slots.append(Slot(SLOT_1Q, [u.comp]))

@@ -9,8 +9,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1,
-                      OP_DAGGER, SLOT_1Q)
+from .planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1, OP_R1, OP_PGRAD,
+                      OP_GFLUSH, OP_DAGGER, SLOT_1Q)
 
 HDR_TILE_BITS = 4
 HDR_HI_OFF = 36
@@ -51,10 +51,14 @@ def _pdep(v, positions):
 def _fwd_op_words(op, slots):
     kind, b0, b1, cmask, si = op
     mat = slots[si].mat_off if si >= 0 else 0
-    if kind == "u1":
-        t = OP_D1 if slots[si].diagonal else OP_U1
-    elif kind == "cu1":
-        t = OP_CD1 if slots[si].diagonal else OP_CU1
+    if kind in ("u1", "cu1"):
+        sl = slots[si]
+        if sl.diagonal:
+            t = OP_D1 if kind == "u1" else OP_CD1
+        elif sl.real:
+            t = OP_R1
+        else:
+            t = OP_U1 if kind == "u1" else OP_CU1
     elif kind == "cx":
         t = OP_CX
     elif kind == "cz":
@@ -69,8 +73,13 @@ def _adj_op_words(op, slots):
     kind, b0, b1, cmask, si = op
     words = []
     if si >= 0 and slots[si].gslot >= 0:
-        g = OP_GRAD1 if kind == "u1" else OP_CGRAD1
-        words.append([g, b0, 0, cmask, 0, slots[si].gslot, 0, 0])
+        sl = slots[si]
+        if sl.scomp >= 0:
+            pk, neg = sl.pauli
+            words.append([OP_PGRAD, b0, 0, cmask, 0, sl.gslot, pk | (int(neg) << 4), sl.scomp])
+        else:
+            g = OP_GRAD1 if kind == "u1" else OP_CGRAD1
+            words.append([g, b0, 0, cmask, 0, sl.gslot, 0, 0])
     w = _fwd_op_words(op, slots)
     w[0] |= OP_DAGGER
     words.append(w)
@@ -94,11 +103,19 @@ def _encode_pass(p, k, r, slots, adjoint):
         base = HDR_STAGES + STAGE_WORDS * s_idx
         ops = []
         seq = reversed(st.ops) if adjoint else st.ops
+        pending = -1          # gradient group collecting OP_PGRAD values
         for op in seq:
             if adjoint:
-                ops.extend(_adj_op_words(op, slots))
+                for w in _adj_op_words(op, slots):
+                    if w[0] == OP_PGRAD and w[5] != pending:
+                        if pending >= 0:
+                            ops.append([OP_GFLUSH, 0, 0, 0, 0, pending, 0, 0])
+                        pending = w[5]
+                    ops.append(w)
             else:
                 ops.append(_fwd_op_words(op, slots))
+        if pending >= 0:
+            ops.append([OP_GFLUSH, 0, 0, 0, 0, pending, 0, 0])
         regmask = sum(1 << b for b in st.reg_bits)
         header[base + 0] = len(ops)
         header[base + 1] = op_start + len(op_words) * OP_WORDS
@@ -137,7 +154,7 @@ def encode_plan(plan):
     comp_list = []
     pgslot = np.full(max(plan.n_params, 1), -1, dtype=np.int32)
     for si, s in enumerate(plan.slots):
-        slot_recs[si] = [s.kind, len(s.comps), len(comp_list), s.mat_off, s.gslot, 0, 0, 0]
+        slot_recs[si] = [s.kind, len(s.comps), len(comp_list), s.mat_off, s.gslot, s.scomp + 1, 0, 0]
         for c in s.comps:
             srcs = list(c.srcs) + [NO_SRC] * (3 - len(c.srcs))
             comp_list.append([c.code, int(c.dagger), srcs[0], srcs[1], srcs[2], c.const_mat, 0, 0])

@@ -108,10 +108,10 @@ class _DeviceBuffers:
 class _Program:
     """One fusion plan (tile bits k, register bits r) with its encoded device program."""
 
-    def __init__(self, n, units, n_params, const_pool, k, r):
+    def __init__(self, n, units, n_params, const_pool, k, r, specialize=False):
         self.n = n
         self.n_params = n_params
-        self.plan = build_plan(n, units, n_params, const_pool, k_max=k, r=r)
+        self.plan = build_plan(n, units, n_params, const_pool, k_max=k, r=r, specialize=specialize)
         self.enc = encode_plan(self.plan)
         self._dev = {}
         self._jit = {}          # device key -> JitModule of generated pass kernels
@@ -275,13 +275,16 @@ class SVExecutor:
             self.enable_jit(device)
             self._jitted.add(key)
 
-    def _program(self, k, r):
-        """Fusion plan for (k, r), built once per executor (with its compiled kernels, if any)."""
+    def _program(self, k, r, adjoint=False):
+        """Fusion plan for (k, r), built once per executor (with its compiled kernels, if any).  The
+        adjoint's plan keeps trainable rotations unfused (single-value gradients, real 2x2 ops)."""
         if not hasattr(self, "_programs"):
             self._programs = {}
-        if (k, r) not in self._programs:
-            self._programs[(k, r)] = _Program(self.n, self._units, self.n_params, list(self._pool), k, r)
-        return self._programs[(k, r)]
+        key = (k, r, adjoint)
+        if key not in self._programs:
+            self._programs[key] = _Program(self.n, self._units, self.n_params, list(self._pool), k, r,
+                                           specialize=adjoint)
+        return self._programs[key]
 
     def configure(self, k, r, k_bwd=None, r_bwd=None):
         """Select the forward and adjoint programs for the given tile / register shapes."""
@@ -290,7 +293,7 @@ class SVExecutor:
         # the adjoint holds psi and lambda registers: more than 16 amplitudes each would spill
         r_bwd = min(4, r if r_bwd is None else r_bwd)
         self.fwd = self._program(k, r)
-        self.bwd = self._program(k_bwd, r_bwd)
+        self.bwd = self._program(k_bwd, r_bwd, adjoint=True)
 
     # ------------------------------------------------------------------
     @property

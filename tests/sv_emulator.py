@@ -14,7 +14,8 @@ import torch
 
 from oracle.sv_oracle import gate_matrix
 from tedq_amd.engine import program as P
-from tedq_amd.engine.planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_U2, OP_GRAD1, OP_CGRAD1, OP_CD1)
+from tedq_amd.engine.planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_U2, OP_GRAD1, OP_CGRAD1, OP_CD1,
+                                     OP_R1, OP_PGRAD, OP_GFLUSH)
 
 CODE_NAMES = {0: "I", 1: "Hadamard", 2: "PauliX", 3: "PauliY", 4: "PauliZ", 5: "S", 6: "T", 7: "SX",
               14: "RX", 15: "RY", 16: "RZ", 17: "Rot", 18: "PhaseShift"}
@@ -66,6 +67,17 @@ def build_tables(enc, plan, row):
         for U in Us:
             S = U @ S
         mats[moff:moff + 4] = S.reshape(-1)
+        scalar = int(rec[5])
+        if scalar > 0:
+            # single-value gradient in component c of the gslot: A picks it out of 2 Re sum A C
+            c = scalar - 1
+            A = np.zeros(4, dtype=np.complex128)
+            A[c >> 1] = -0.5j if c & 1 else 0.5
+            for comp in comps:
+                for src in (int(x) for x in comp[2:5]):
+                    if src != P.NO_SRC and src >= 0:
+                        amats[src] = A.reshape(2, 2)
+            continue
         prev = np.eye(2, dtype=np.complex128)
         for c, U in zip(comps, Us):
             cur = U @ prev
@@ -92,6 +104,31 @@ def _apply_ops(regs_list, ops, mats, acc):
     for op in ops:
         typ, dag = int(op[0]) & 0xFF, (int(op[0]) >> 8) & 1
         b0, b1, cmask, moff, gslot = int(op[1]), int(op[2]), int(op[3]), int(op[4]), int(op[5])
+        if typ == OP_GFLUSH:
+            continue
+        if typ == OP_PGRAD:
+            p, l = regs_list
+            kind, neg, comp = int(op[6]) & 15, int(op[6]) >> 4, int(op[7])
+            val = 0.0
+            for i in range(R):
+                if i & (1 << b0) or (i & cmask) != cmask:
+                    continue
+                j = i | (1 << b0)
+                c = lambda a, b: np.sum(np.conj(l[:, a]) * p[:, b])  # noqa: E731
+                if kind == 0:
+                    val += (c(i, j) + c(j, i)).imag
+                elif kind == 1:
+                    val += (c(j, i) - c(i, j)).real
+                elif kind == 2:
+                    val += (c(i, i) - c(j, j)).imag
+                else:
+                    val += -2.0 * c(j, j).imag
+            if neg:
+                val = -val
+            cm = np.zeros(4, dtype=np.complex128)
+            cm[comp >> 1] = val if comp % 2 == 0 else 1j * val
+            acc[gslot] = acc.get(gslot, 0) + cm.reshape(2, 2)
+            continue
         if typ in (OP_GRAD1, OP_CGRAD1):
             p, l = regs_list
             c = np.zeros((2, 2), dtype=np.complex128)
@@ -106,10 +143,13 @@ def _apply_ops(regs_list, ops, mats, acc):
             acc[gslot] = acc.get(gslot, 0) + c
             continue
         for v in regs_list:
-            if typ in (OP_U1, OP_CU1, OP_D1, OP_CD1):
+            if typ in (OP_U1, OP_CU1, OP_D1, OP_CD1, OP_R1):
                 m = mats[moff:moff + 4].reshape(2, 2)
                 if typ in (OP_D1, OP_CD1):
                     m = np.diag(np.diag(m))
+                if typ == OP_R1:
+                    assert np.allclose(m.imag, 0)
+                    m = m.real.astype(np.complex128)
                 if dag:
                     m = m.conj().T
                 for i in range(R):

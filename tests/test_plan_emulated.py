@@ -78,8 +78,9 @@ def test_emulated_forward_matches_oracle(seed, n, k, r):
         assert p.tile_bits == sorted(set(p.tile_bits))
 
 
+@pytest.mark.parametrize("specialize", [False, True])
 @pytest.mark.parametrize("seed,n,k,r", [(10, 5, 3, 2), (11, 6, 4, 3), (12, 7, 7, 4), (13, 8, 5, 2)])
-def test_emulated_adjoint_gradient_matches_autograd(seed, n, k, r):
+def test_emulated_adjoint_gradient_matches_autograd(seed, n, k, r, specialize):
     rng = np.random.default_rng(seed)
     spec = random_spec(n, 30, rng)
     spec.append(("Rot", [0], 3))
@@ -90,8 +91,13 @@ def test_emulated_adjoint_gradient_matches_autograd(seed, n, k, r):
                                            qai.expval(qai.PauliZ(qubits=[obs_q[1]]))])
     cc = circ.compilecircuit(backend="pytorch")
     nq, units, P, pool = lower_circuit(cc)
-    plan = build_plan(nq, units, P, pool, k_max=k, r=r)
+    plan = build_plan(nq, units, P, pool, k_max=k, r=r, specialize=specialize)
     enc = encode_plan(plan)
+    if specialize:
+        # every RX/RY/RZ/PhaseShift (plain or controlled) gets a single-value gradient
+        n_rot = sum(1 for name, _, _ in spec if name in ("RX", "RY", "RZ", "PhaseShift", "CRX", "CRY", "CRZ",
+                                                            "ControlledPhaseShift"))
+        assert sum(1 for sl in plan.slots if sl.scomp >= 0) == n_rot
     row = np.asarray(values)
     psi, mats, amats = EMU.forward(enc, plan, row)
     dy = rng.normal(size=2)
@@ -176,3 +182,46 @@ def test_two_qubit_unitary_lowering():
     ref = O.propagate(n, [("Hadamard", [0], []), ("RY", [2], [0.3]), ("Unitary", [2, 0], U1),
                           ("Unitary", [1, 3], U2.conj().T), ("Unitary", [3], u1q), ("CNOT", [3, 1], [])])
     assert np.allclose(psi, ref.numpy().reshape(-1), atol=1e-12)
+
+
+def test_specialized_adjoint_with_inverted_and_controlled_rotations():
+    """The adjoint plan's single-value gradients (RX, RY, RZ, PhaseShift, their controlled blocks and
+    their inverses, which negate the value) and real 2x2 ops give the oracle's autograd gradient; more
+    than 8 such gradients in one register stage span several groups."""
+    n = 4
+    rng = np.random.default_rng(7)
+    names = ["RX", "RY", "RZ", "PhaseShift", "CRX", "CRY", "CRZ", "ControlledPhaseShift", "Hadamard", "CNOT"]
+    spec, inv = [], []
+    for _ in range(40):
+        name = names[rng.integers(len(names))]
+        k = 2 if name.startswith("C") else 1
+        spec.append((name, [int(q) for q in rng.choice(n, size=k, replace=False)], O.PARAM_COUNT.get(name, 0)))
+        inv.append(bool(rng.integers(2)) and spec[-1][2] > 0)
+    npar = sum(x[2] for x in spec)
+    values = [float(x) for x in rng.uniform(-math.pi, math.pi, npar)]
+
+    def circuit_def():
+        it = iter(values)
+        for (name, qubits, k), iv in zip(spec, inv):
+            g = getattr(qai, name)(*[next(it) for _ in range(k)], qubits=qubits)
+            if iv:
+                g.inverse()
+        return qai.expval(qai.PauliZ(qubits=[1]))
+    cc = qai.Circuit(circuit_def, n).compilecircuit(backend="pytorch")
+    nq, units, P, pool = lower_circuit(cc)
+    plan = build_plan(nq, units, P, pool, k_max=4, r=4, specialize=True)
+    enc = encode_plan(plan)
+    assert sum(1 for sl in plan.slots if sl.scomp >= 0) == npar
+    assert max(sum(1 for o in st.ops if o[4] >= 0 and plan.slots[o[4]].scomp >= 0)
+               for p in plan.passes for st in p.stages) > 8
+    psi, mats, amats = EMU.forward(enc, plan, np.asarray(values))
+    lam = (1 - 2 * ((np.arange(2 ** n) >> (n - 2)) & 1)) * psi
+    grad, _ = EMU.adjoint_grad(enc, plan, psi, lam, mats, amats, P)
+    t = torch.tensor(values, dtype=torch.float64, requires_grad=True)
+    it = iter(t)
+    gates = []
+    for (name, qubits, k), iv in zip(spec, inv):
+        ps = [next(it) for _ in range(k)]
+        gates.append((name, qubits, [-x for x in ps] if iv else ps))
+    O.execute(n, gates, [("expval", ("PauliZ", [1]))]).sum().backward()
+    assert np.allclose(grad, t.grad.numpy(), atol=1e-9)
