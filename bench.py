@@ -277,8 +277,11 @@ def serving_stream(n_req, seed=4, rate=None):
                     float(t[i])) for i, (n, L) in enumerate(zip(ns, ls))]
 
 
-def serving_leg(n_req, warm):
-    """Closed-loop capacity, Poisson stream at 0.7x capacity (p50/p99), and the unbatched path."""
+def serving_leg(n_req, warm, ragged=True):
+    """Closed-loop capacity, Poisson stream at 0.7x capacity (p50/p99), and the unbatched path.
+
+    With ``ragged`` the engine batches requests of different structures that share a tile shape into
+    one launch per pass index; without it only identical structures share launches."""
     from tedq_amd.serving.engine import ServingEngine, latency_stats  # pylint: disable=import-outside-toplevel
     compiled = {}
 
@@ -287,7 +290,7 @@ def serving_leg(n_req, warm):
             compiled[key] = hea_z0_circuit(*key).compilecircuit(backend="pytorch")
         return compiled[key]
 
-    eng = ServingEngine(factory, hbm_budget=64 << 30, max_batch=64, n_streams=4, max_inflight=8)
+    eng = ServingEngine(factory, hbm_budget=64 << 30, max_batch=64, n_streams=4, max_inflight=8, ragged=ragged)
     closed = serving_stream(n_req)
     keys = sorted({r.key for r in closed} | {r.key for r in serving_stream(n_req, seed=5, rate=1.0)})
     eng.prepare(keys, buckets=(1, 2, 4), log=progress)
@@ -296,15 +299,22 @@ def serving_leg(n_req, warm):
     eng.batch_sizes.clear()
     eng.run(closed)
     cap = latency_stats(closed)
-    progress(f"serving: closed-loop capacity {cap['evals_per_s']:.0f} evals/s")
+    closed_batch = float(np.mean(eng.batch_sizes))
+    progress(f"serving ({'ragged' if ragged else 'same-structure'}): closed-loop capacity "
+             f"{cap['evals_per_s']:.0f} evals/s, mean batch {closed_batch:.1f}")
     rate = 0.7 * cap["evals_per_s"]
     eng.completed.clear()
     eng.batch_sizes.clear()
     stream = serving_stream(n_req, seed=5, rate=rate)
     eng.run(stream)
     st = latency_stats(stream)
-    progress(f"serving: Poisson stream p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
     mean_batch = float(np.mean(eng.batch_sizes))
+    progress(f"serving: Poisson stream p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, "
+             f"mean batch {mean_batch:.2f}")
+    if not ragged:
+        return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "evals_per_s": st["evals_per_s"],
+                "capacity_evals_per_s": cap["evals_per_s"], "mean_batch": mean_batch,
+                "closed_loop_mean_batch": closed_batch}
     single = ServingEngine(factory, hbm_budget=64 << 30, max_batch=1, n_streams=1, max_inflight=1,
                            graph_max_qubits=0)
     sub = serving_stream(min(n_req, 400), seed=6)
@@ -314,7 +324,10 @@ def serving_leg(n_req, warm):
     return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "evals_per_s": st["evals_per_s"], "arrival_rate": rate, "requests": st["requests"],
             "capacity_evals_per_s": cap["evals_per_s"], "mean_batch": mean_batch,
+            "closed_loop_mean_batch": closed_batch, "ragged_batches": eng.ragged_batches,
             "unbatched_evals_per_s": un["evals_per_s"], "graphs_captured": len(eng._graphs),
+            "batching": "ragged: requests of different structures sharing a tile shape run in one launch "
+                        "per pass index; identical structures also via captured graphs",
             "stream": "Poisson at 0.7x closed-loop capacity; n~U{6..24}, layers~U{2..20}; <Z_0> + full gradient"}
 
 
@@ -558,7 +571,8 @@ def main():
                                "TB_per_s": value * (3 * L * (3 * n - 1) + 2 * n * L + 1) * 16 * 2 ** n / 1e12},
     }
     if not args.skip_serving:
-        line["latency"] = serving_leg(args.serve_requests, warm=200)
+        line["latency"] = serving_leg(args.serve_requests, warm=200, ragged=True)
+        line["latency"]["same_structure_only"] = serving_leg(args.serve_requests, warm=200, ragged=False)
     if not args.skip_tn:
         line["tensor_network"] = tn_leg(c4=not args.skip_c4)
     if not args.skip_plain:

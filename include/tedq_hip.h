@@ -67,6 +67,19 @@ int tedq_sv_apply_pass(void* state, int64_t batch, int n_qubits, int tile_bits, 
 int tedq_sv_adjoint_pass(void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits, int reg_bits,
                          int threads_log2, const int* dev_prog, uint64_t tile_mask, const void* mats,
                          int64_t mat_stride, float* partials, int64_t partial_rows, int n_gslots, void* stream);
+/* One pass for circuits of DIFFERENT structures and sizes in one launch (a
+ * serving batch), forward (adjoint = 0) or adjoint.  dev_recs: device array of
+ * n_recs 64-byte records {psi, lam, prog, mats, partials (device pointers of
+ * the circuit: state, adjoint state, its program of this pass, its slot-matrix
+ * row, its partial rows [n_gslots][rows][8]), uint64 rest_mask (the circuit's
+ * non-tile bits in this pass), int64 tile_end (tiles of this and all earlier
+ * records, 2^(n-k) per circuit), int32 n, int32 n_gslots}; all circuits share
+ * tile bits k and register bits r; total_tiles = the last tile_end.  Each
+ * circuit is processed exactly as by tedq_sv_apply_pass / tedq_sv_adjoint_pass
+ * with batch 1.  Replaces, for a batch of different circuits, the same
+ * reference SV loop (pytorch_backend.py:358-380) and its autograd backward. */
+int tedq_sv_pass_ragged(int adjoint, const void* dev_recs, int n_recs, int64_t total_tiles, int k, int r,
+                        void* stream);
 /* grad[b][p] = 2 Re sum_ij A[b][p][ij] C[b][g(p)][ij] with C reduced in fp64
  * (compensated) from the partial rows; grad is float32 or float64
  * (grad_is_f64).  pgslot: dev int32 [n_params]; C: dev double scratch

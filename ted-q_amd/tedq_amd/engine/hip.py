@@ -25,6 +25,7 @@ _SIGNATURES = {
     "tedq_sv_apply_pass": ([c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_u64, c_vp, c_i64, c_vp], c_int),
     "tedq_sv_adjoint_pass": ([c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_u64, c_vp, c_i64, c_vp, c_i64,
                               c_int, c_vp], c_int),
+    "tedq_sv_pass_ragged": ([c_int, c_vp, c_int, c_i64, c_int, c_int, c_vp], c_int),
     "tedq_sv_init_zero": ([c_vp, c_i64, c_int, c_vp], c_int),
     "tedq_sv_init_copy": ([c_vp, c_vp, c_i64, c_int, c_vp], c_int),
     "tedq_sv_reduce_chunks": ([c_int], c_i64),

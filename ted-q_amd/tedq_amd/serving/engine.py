@@ -8,6 +8,10 @@ the circuit's measurements and the full parameter gradient.  The loop
 * queues requests per key as they arrive and forms batches of structurally
   identical circuits from whatever is waiting when a slot frees up, oldest
   request first;
+* with ``ragged=True`` queues by tile shape instead: a batch may mix circuit
+  structures (depths, qubit counts from 12 up), and every pass index of all
+  of them runs as one ragged launch (tedq_amd/engine/ragged.py), while a batch
+  that happens to hold one structure takes the same-structure path;
 * admits a batch only while the bytes of all in-flight batches stay within an
   HBM budget (requests wait otherwise);
 * runs batches on several HIP streams so their kernels overlap on the device,
@@ -92,6 +96,8 @@ class _Inflight:
     host_vals: torch.Tensor
     host_grad: torch.Tensor
     keep: list = field(default_factory=list)
+    # ragged batches: per request (host values, host gradients, row) instead of the two tensors above
+    slots: list = None
 
 
 class ServingEngine:
@@ -105,8 +111,11 @@ class ServingEngine:
     """
 
     def __init__(self, factory, device=None, hbm_budget=32 << 30, max_batch=64, n_streams=4, max_inflight=8,
-                 graph_max_qubits=18):
+                 graph_max_qubits=18, ragged=False):
         self.factory = factory
+        # queue by tile shape and mix structures in one launch sequence (see module docstring)
+        self.ragged = ragged
+        self.ragged_batches = 0
         self.graph_max_qubits = graph_max_qubits
         self._graphs = {}
         self._graph_busy = set()
@@ -130,8 +139,14 @@ class ServingEngine:
             self._compiled[key] = self.factory(key).executor()
         return self._compiled[key]
 
+    def _queue_key(self, key):
+        if not self.ragged:
+            return key
+        from ..engine.ragged import shape_key  # pylint: disable=import-outside-toplevel
+        return ("shape",) + shape_key(self.executor(key))
+
     def submit(self, req):
-        self._queues.setdefault(req.key, deque()).append(req)
+        self._queues.setdefault(self._queue_key(req.key), deque()).append(req)
 
     @property
     def pending(self):
@@ -184,8 +199,8 @@ class ServingEngine:
                 log(f"prepared {i + 1}/{len(keys)} circuit structures")
         torch.cuda.synchronize(self.device)
 
-    def _launch_graph(self, key, ex, now):
-        q = self._queues[key]
+    def _launch_graph(self, key, ex, now, q=None):
+        q = self._queues[key] if q is None else q
         b = min(len(q), self.max_batch)
         gk = self._graph_for(key, ex, b)
         if gk is None:
@@ -200,12 +215,21 @@ class ServingEngine:
         self.batch_sizes.append(b)
         return True
 
-    def _launch(self, key, now):
+    def _launch(self, qkey, now):
+        q = self._queues[qkey]
+        if self.ragged:
+            head = [q[i].key for i in range(min(len(q), self.max_batch))]
+            if len(set(head)) > 1:
+                return self._launch_ragged(qkey, now)
+            key = head[0]
+        else:
+            key = qkey
         ex = self.executor(key)
-        if ex.n <= self.graph_max_qubits and self._launch_graph(key, ex, now):
+        if ex.n <= self.graph_max_qubits and self._launch_graph(key, ex, now, q):
             return True
-        q = self._queues[key]
         B = min(len(q), self.max_batch)
+        if self.ragged:
+            B = min(B, sum(1 for r in q if r.key == key)) if q[0].key == key else 1
         while B > 1 and self._bytes_in_use + self.graph_bytes + ex.memory_bytes(B) > self.hbm_budget:
             B //= 2
         need = ex.memory_bytes(B)
@@ -233,6 +257,57 @@ class ServingEngine:
         self.batch_sizes.append(B)
         return True
 
+    def _launch_ragged(self, qkey, now):
+        """One launch sequence for the oldest waiting requests of a tile shape, whatever their structures."""
+        from ..engine.ragged import ragged_value_and_grad  # pylint: disable=import-outside-toplevel
+        q = self._queues[qkey]
+        reqs, need = [], 0
+        while q and len(reqs) < self.max_batch:
+            more = self.executor(q[0].key).memory_bytes(1)
+            if reqs and self._bytes_in_use + self.graph_bytes + need + more > self.hbm_budget:
+                break
+            reqs.append(q.popleft())
+            need += more
+        if self._bytes_in_use + self.graph_bytes + need > self.hbm_budget and self._inflight:
+            for r in reversed(reqs):
+                q.appendleft(r)
+            return False
+        groups = {}
+        for r in reqs:
+            groups.setdefault(r.key, []).append(r)
+        stream = self.streams[self._next_stream % len(self.streams)]
+        self._next_stream += 1
+        keep, slots, order = [], {}, []
+        with torch.cuda.stream(stream):
+            items = []
+            for key, grs in groups.items():
+                ex = self.executor(key)
+                host = torch.from_numpy(np.stack([r.params for r in grs]).astype(np.float32)).pin_memory()
+                rows = host.to(self.device, non_blocking=True)
+                keep += [host, rows]
+                items.append((ex, rows, [torch.ones(len(grs), device=self.device) for _ in ex.specs]))
+            results, bufs = ragged_value_and_grad(items, self.device)
+            keep += bufs
+            for (key, grs), (outs, grad) in zip(groups.items(), results):
+                vals = torch.stack([o.reshape(len(grs)) for o in outs], 1).to(torch.float32)
+                host_vals = torch.empty(vals.shape, dtype=torch.float32, pin_memory=True)
+                host_grad = torch.empty(grad.shape, dtype=grad.dtype, pin_memory=True)
+                host_vals.copy_(vals, non_blocking=True)
+                host_grad.copy_(grad, non_blocking=True)
+                keep += [vals, grad]
+                for i, r in enumerate(grs):
+                    slots[r.rid] = (host_vals, host_grad, i)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        for r in reqs:
+            r.t_launch = now
+            order.append(slots[r.rid])
+        self._inflight.append(_Inflight(reqs, ev, need, None, None, keep=keep, slots=order))
+        self._bytes_in_use += need
+        self.batch_sizes.append(len(reqs))
+        self.ragged_batches += 1
+        return True
+
     def poll(self):
         """Retire finished batches; returns the number of requests completed."""
         done = 0
@@ -242,8 +317,9 @@ class ServingEngine:
             if b.event.query():
                 t = time.perf_counter()
                 for i, r in enumerate(b.reqs):
-                    r.value = b.host_vals[i].numpy().copy()
-                    r.grad = b.host_grad[i].numpy().copy()
+                    hv, hg, row = (b.host_vals, b.host_grad, i) if b.slots is None else b.slots[i]
+                    r.value = hv[row].numpy().copy()
+                    r.grad = hg[row].numpy().copy()
                     r.t_done = t
                     self.completed.append(r)
                 self._bytes_in_use -= b.nbytes

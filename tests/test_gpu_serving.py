@@ -147,3 +147,69 @@ def test_large_requests_alone_equal_results_in_the_mix():
     assert abs(float(ref) - float(r.value[0])) <= 1e-5
     g = t.grad.numpy().reshape(-1)
     assert np.max(np.abs(g - r.grad)) / np.max(np.abs(g)) <= 1e-5
+
+
+def test_ragged_launches_equal_circuits_alone():
+    """tedq_sv_pass_ragged: circuits of different depths and qubit counts (12-20 qubits, plus repeated
+    structures) share one launch per pass index; every value and gradient equals the circuit evaluated
+    alone (2e-6) and one equals the oracle (1e-5 relative)."""
+    from tedq_amd.engine.ragged import ragged_value_and_grad
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(21)
+    keys = [(12, 3), (16, 6), (20, 2), (14, 1), (16, 6), (13, 5), (20, 4)]
+    cache = {}
+    items, params = [], []
+    for key in keys:
+        if key not in cache:
+            cache[key] = hea_z0(*key).executor()
+        ex = cache[key]
+        n, L = key
+        B = int(rng.integers(1, 4))
+        p = rng.uniform(0, 2 * np.pi, size=(B, 2 * n * L)).astype(np.float32)
+        params.append(p)
+        items.append((ex, torch.tensor(p, device=dev), [torch.ones(B, device=dev)]))
+    print("ragged batch:", [(k, p.shape[0]) for k, p in zip(keys, params)], flush=True)
+    results, keep = ragged_value_and_grad(items, dev)
+    torch.cuda.synchronize()
+    del keep
+    for (ex, rows, ups), (outs, grad) in zip(items, results):
+        a_outs, a_grad = ex.value_and_grad(rows, ups)
+        assert np.max(np.abs(outs[0].cpu().numpy() - a_outs[0].cpu().numpy())) <= 2e-6
+        assert np.max(np.abs(grad.cpu().numpy() - a_grad.cpu().numpy())) <= 2e-6
+    n, L = keys[1]
+    t = torch.tensor(params[1][0].reshape(L, n, 2), dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, O.hea_gates(n, t)[0], [("expval", ("PauliZ", [0]))])
+    ref.sum().backward()
+    assert abs(float(ref) - float(results[1][0][0][0])) <= 1e-5
+    g = t.grad.numpy().reshape(-1)
+    assert np.max(np.abs(g - results[1][1][0].cpu().numpy())) / np.max(np.abs(g)) <= 1e-5
+
+
+def test_ragged_serving_mixes_structures_and_matches_alone():
+    """ServingEngine(ragged=True): requests of many structures (12-18 qubits, 1-6 layers) are batched
+    together by tile shape; batches hold several structures and every result equals the request
+    evaluated alone (2e-6)."""
+    rng = np.random.default_rng(13)
+    cache = {}
+
+    def factory(key):
+        if key not in cache:
+            cache[key] = hea_z0(*key)
+        return cache[key]
+
+    reqs = []
+    for i in range(80):
+        n, L = int(rng.integers(12, 19)), int(rng.integers(1, 7))
+        reqs.append(Request(i, (n, L), rng.uniform(0, 2 * np.pi, 2 * n * L).astype(np.float32), 0.0))
+    eng = ServingEngine(factory, hbm_budget=8 << 30, max_batch=16, n_streams=2, max_inflight=4, ragged=True)
+    eng.run(reqs)
+    assert len(eng.completed) == len(reqs)
+    print(f"ragged batches {eng.ragged_batches}, batch sizes {eng.batch_sizes}", flush=True)
+    assert eng.ragged_batches > 0 and max(eng.batch_sizes) > 4
+    dev = torch.device("cuda", 0)
+    for r in reqs:
+        ex = factory(r.key).executor()
+        row = torch.tensor(r.params[None, :], device=dev)
+        outs, grad = ex.value_and_grad(row, [torch.ones(1, device=dev)])
+        assert np.max(np.abs(outs[0].float().cpu().numpy().reshape(-1) - r.value)) <= 2e-6
+        assert np.max(np.abs(grad.cpu().numpy().reshape(-1) - r.grad)) <= 2e-6
