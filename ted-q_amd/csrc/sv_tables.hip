@@ -232,8 +232,10 @@ __global__ void k_build_tables(const PT* __restrict__ params, int64_t batch, int
 // builds the same operands with torch ops per gate per call
 // (tedq/backends/pytorch_backend.py:524-546 for the conjugated copies).
 // ---------------------------------------------------------------------------
+// diag = 1: the tensor is stored as its D diagonal entries (a diagonal gate whose wires the network
+// merged into hyper-indices) instead of the D x D matrix.
 struct TNGateRec {
-  int code, src0, src1, src2, const_off, nq, kind, bra, off, d0, d1, d2, dagger, pad1, pad2, pad3;
+  int code, src0, src1, src2, const_off, nq, kind, bra, off, d0, d1, d2, dagger, diag, pad2, pad3;
 };
 
 template <typename PT>
@@ -249,6 +251,13 @@ __global__ void k_tn_gate_tensors(const PT* __restrict__ params, int64_t batch, 
   float2* out = leaves + b * leaf_stride + r.off;
   const int D = 1 << r.nq;
   if (r.kind == 1) {  // constant matrix, row-major complex128
+    if (r.diag) {
+      for (int i = 0; i < D; ++i) {
+        const double2 v = cpool[r.const_off + i * D + i];
+        out[i] = make_float2((float)v.x, (float)(r.bra ? -v.y : v.y));
+      }
+      return;
+    }
     for (int i = 0; i < D; ++i)
       for (int j = 0; j < D; ++j) {
         const double2 v = r.bra ? cpool[r.const_off + j * D + i] : cpool[r.const_off + i * D + j];
@@ -267,7 +276,14 @@ __global__ void k_tn_gate_tensors(const PT* __restrict__ params, int64_t batch, 
   const M2 V = comp_eval(cr, prow, cpool, -1, &ok);
   if (!ok) atomicOr(err, 2);
   // controlled embedding: identity except the block where every control is 1
-  for (int i = 0; i < D; ++i)
+  if (r.diag) {
+    for (int i = 0; i < D; ++i) {
+      const int bi = i - (D - 2);
+      const cd v = bi >= 0 ? (r.bra ? cd_conj(V.a[bi * 3]) : V.a[bi * 3]) : mkcd(1.0, 0.0);
+      out[i] = make_float2((float)v.x, (float)v.y);
+    }
+  }
+  for (int i = 0; i < D && !r.diag; ++i)
     for (int j = 0; j < D; ++j) {
       const int bi = i - (D - 2), bj = j - (D - 2);
       cd v;
@@ -305,8 +321,25 @@ __global__ void k_tn_param_grad(const float2* __restrict__ env, int64_t env_stri
   const TNParamRec r = recs[p];
   if (r.dslot < 0) return;
   const float2* dv = dmats + b * dmat_stride + (int64_t)r.dslot * 4;
-  const int D = 1 << r.nq;
+  const bool diag = r.nq & 16;   // leaves stored as diagonals: entry (i, i) at position i
+  const int D = 1 << (r.nq & 15);
   double acc = 0.0;
+  if (diag) {
+    for (int a = 0; a < 2; ++a) {
+      const int e = D - 2 + a;
+      const float2 d = dv[a * 2 + a];
+      if (r.ket_off >= 0) {
+        const float2 x = env[b * env_stride + r.ket_off + e];
+        acc += (double)x.x * d.x - (double)x.y * d.y;
+      }
+      if (r.bra_off >= 0) {
+        const float2 x = env[b * env_stride + r.bra_off + e];
+        acc += (double)x.x * d.x + (double)x.y * d.y;
+      }
+    }
+    grad[b * np + p] += (double)upstream[b] * acc;
+    return;
+  }
   for (int a = 0; a < 2; ++a)
     for (int c = 0; c < 2; ++c) {
       const int e = (D - 2 + a) * D + (D - 2 + c);

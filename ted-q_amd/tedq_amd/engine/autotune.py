@@ -52,7 +52,7 @@ def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2, jit
     for k, r in fwd_shapes:
         ex.configure(k, r, kb0, rb0)
         if jit:
-            ex.fwd.enable_jit(dev, True, False)
+            ex.enable_jit(dev, forward=True, adjoint=False)
         fwd_ms[(k, r)] = _time(lambda: ex.run_forward(rows), reps)
         if log:
             log(f"autotune forward {k}/{r}: {fwd_ms[(k, r)]:.2f} ms")
@@ -64,7 +64,7 @@ def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2, jit
     for k, r in bwd_shapes:
         ex.configure(kf, rf, k, r)
         if jit:
-            ex.bwd.enable_jit(dev, False, True)
+            ex.enable_jit(dev, forward=False, adjoint=True)
         bwd_ms[(k, r)] = _time(lambda: ex.run_backward(rows, psi.clone(), lam.clone()), reps)
         if log:
             log(f"autotune adjoint {k}/{r}: {bwd_ms[(k, r)]:.2f} ms")

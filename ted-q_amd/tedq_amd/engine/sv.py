@@ -37,6 +37,11 @@ class MeasSpec:
     fbits: list = field(default_factory=list)     # factored observable: physical bits
     ffactors: np.ndarray = None                    # [q, 2, 2] complex
     pbits: list = field(default_factory=list)     # probability bits, MSB first (empty: all)
+    # more than 4 non-Z factors, all Hermitian: F_i = V_i (a_i I + b_i Z) V_i^dagger, so the value is
+    # sum_S c_S <phi| Z_{m_S} |phi> with phi = (x V_i^dagger) psi (basis: [(bit, V_i)], bmasks / bcoefs)
+    basis: list = None
+    bmasks: list = None
+    bcoefs: list = None
 
 
 def _observable_factors(n, meas):
@@ -50,6 +55,34 @@ def _observable_factors(n, meas):
         m = np.asarray(ob.matrix, dtype=np.complex128).reshape(2, 2)
         factors[bit] = m @ factors.get(bit, _I)
     return factors
+
+
+def _basis_spec(zmask, rest):
+    """Product observable with more than 4 non-Z single-qubit factors, each Hermitian: diagonalise every
+    factor, F = V (a I + b Z) V^dagger (a, b from its eigenvalues), and expand the product of the
+    (a I + b Z) into Z-strings with real coefficients (terms with a zero coefficient dropped; a Pauli
+    factor has a = 0 and contributes one term)."""
+    basis, terms = [], [(zmask, 1.0)]
+    for bit in sorted(rest):
+        f = rest[bit]
+        if not np.allclose(f, f.conj().T, atol=1e-12):
+            raise NotImplementedError("expval over more than 4 non-Z qubits needs Hermitian factors")
+        w, v = np.linalg.eigh(f)
+        a, b = 0.5 * (w[1] + w[0]), 0.5 * (w[1] - w[0])
+        # eigh sorts ascending: put the larger eigenvalue on |0> so that the Z coefficient is b
+        v = v[:, ::-1]
+        basis.append((bit, v))
+        nxt = []
+        for m, c in terms:
+            if abs(a) > 1e-15:
+                nxt.append((m, c * a))
+            if abs(b) > 1e-15:
+                nxt.append((m | (1 << bit), c * b))
+        terms = nxt
+    if len(terms) > ZMAX:
+        raise NotImplementedError(f"expval expands into {len(terms)} Z-strings (at most {ZMAX})")
+    return MeasSpec("fexp", (), zmask=zmask, basis=basis, bmasks=[m for m, _ in terms],
+                    bcoefs=[c for _, c in terms])
 
 
 def compile_measurements(n, measurements):
@@ -69,12 +102,12 @@ def compile_measurements(n, measurements):
             if not rest:
                 specs.append(MeasSpec("zexp", (), zindex=len(zmasks), zmask=zmask))
                 zmasks.append(zmask)
-            else:
-                if len(rest) > 4:
-                    raise NotImplementedError("expval over more than 4 non-Z qubits is not supported")
+            elif len(rest) <= 4:
                 bits = sorted(rest)
                 specs.append(MeasSpec("fexp", (), zmask=zmask, fbits=bits,
                                       ffactors=np.stack([rest[b] for b in bits])))
+            else:
+                specs.append(_basis_spec(zmask, rest))
         elif meas.return_type is Probability:
             qs = sorted(set(meas.qubits)) if meas.qubits is not None else list(range(n))
             # every qubit in ascending order is the whole distribution (no marginal sum)
@@ -258,14 +291,23 @@ class SVExecutor:
         self._jitted = set()
         self.configure(k_max, r, k_bwd, r_bwd)
 
-    def enable_jit(self, device=None):
+    def enable_jit(self, device=None, forward=True, adjoint=True):
         """Compile the forward and adjoint programs into per-pass kernels (hiprtc) for ``device``.
 
-        Returns (forward_jitted, adjoint_jitted).  Compiled programs are kept per (k, r) shape.
+        The adjoint switches to its specialised plan (unfused rotations with single-value gradients)
+        when generated kernels can run it: that plan has twice the ops, which only generated code
+        executes without per-op overhead.  Returns (forward_jitted, adjoint_jitted).  Compiled
+        programs are kept per (k, r) shape.
         """
+        from . import jit  # pylint: disable=import-outside-toplevel
         device = device or torch.device("cuda", torch.cuda.current_device())
-        self.fwd.enable_jit(device, True, False)
-        self.bwd.enable_jit(device, False, True)
+        if forward:
+            self.fwd.enable_jit(device, True, False)
+        if adjoint:
+            k, r = self.bwd.plan.k, self.bwd.plan.r
+            if jit.supported(self.n, k, r, True):
+                self.bwd = self._program(k, r, adjoint=True, specialize=True)
+            self.bwd.enable_jit(device, False, True)
         # pylint: disable=protected-access
         return self.fwd._jit_fns(device, False) is not None, self.bwd._jit_fns(device, True) is not None
 
@@ -275,15 +317,16 @@ class SVExecutor:
             self.enable_jit(device)
             self._jitted.add(key)
 
-    def _program(self, k, r, adjoint=False):
+    def _program(self, k, r, adjoint=False, specialize=False):
         """Fusion plan for (k, r), built once per executor (with its compiled kernels, if any).  The
-        adjoint's plan keeps trainable rotations unfused (single-value gradients, real 2x2 ops)."""
+        specialised adjoint plan keeps trainable rotations unfused (single-value gradients, real 2x2
+        ops) for generated kernels."""
         if not hasattr(self, "_programs"):
             self._programs = {}
-        key = (k, r, adjoint)
+        key = (k, r, adjoint and specialize)
         if key not in self._programs:
             self._programs[key] = _Program(self.n, self._units, self.n_params, list(self._pool), k, r,
-                                           specialize=adjoint)
+                                           specialize=adjoint and specialize)
         return self._programs[key]
 
     def configure(self, k, r, k_bwd=None, r_bwd=None):
@@ -294,6 +337,8 @@ class SVExecutor:
         r_bwd = min(4, r if r_bwd is None else r_bwd)
         self.fwd = self._program(k, r)
         self.bwd = self._program(k_bwd, r_bwd, adjoint=True)
+        if self.jit:
+            self._ensure_jit(torch.device("cuda", torch.cuda.current_device()))
 
     # ------------------------------------------------------------------
     @property
@@ -366,6 +411,15 @@ class SVExecutor:
         for spec in self.specs:
             if spec.kind == "zexp":
                 outs.append(zout[:, spec.zindex])
+            elif spec.kind == "fexp" and spec.basis is not None:
+                phi = self._basis_apply(spec, psi, inverse=True)
+                M = len(spec.bmasks)
+                masks = self._basis_masks(spec, dev)
+                part = torch.empty((B, nchunk, M), dtype=torch.float64, device=dev)
+                vals = torch.empty((B, M), dtype=torch.float64, device=dev)
+                hip.call("tedq_sv_expval_z", hip.ptr(phi), B, self.n, hip.ptr(masks), M, hip.ptr(part),
+                         hip.ptr(vals), st)
+                outs.append(vals @ torch.tensor(spec.bcoefs, dtype=torch.float64, device=dev))
             elif spec.kind == "fexp":
                 bits, fac = _factored_args(spec, dev)
                 partial = torch.empty((B, nchunk), dtype=torch.float64, device=dev)
@@ -388,6 +442,33 @@ class SVExecutor:
             else:
                 outs.append(psi.view((B,) + spec.shape))
         return outs
+
+    def _basis_program(self, spec, inverse):
+        """Pass program applying V_i^dagger (inverse) or V_i to every factor bit of a basis spec."""
+        from .planner import Unit, _const_comp  # pylint: disable=import-outside-toplevel
+        if not hasattr(self, "_basis_progs"):
+            self._basis_progs = {}
+        key = (id(spec), inverse)
+        if key not in self._basis_progs:
+            pool = []
+            units = [Unit("u1", (bit,), _const_comp(v.conj().T if inverse else v, pool)) for bit, v in spec.basis]
+            self._basis_progs[key] = _Program(self.n, units, 0, pool, self.fwd.plan.k, self.fwd.plan.r)
+        return self._basis_progs[key]
+
+    def _basis_masks(self, spec, device):
+        key = ("bmask", id(spec), device.type, device.index)
+        if key not in self._dev:
+            self._dev[key] = torch.tensor(np.asarray(spec.bmasks, dtype=np.uint64).view(np.int64), device=device)
+        return self._dev[key]
+
+    def _basis_apply(self, spec, psi, inverse, in_place=False):
+        """(x V^dagger) psi (inverse) or (x V) psi over the factor bits, on a copy unless in_place."""
+        prog = self._basis_program(spec, inverse)
+        out = psi if in_place else psi.clone()
+        B = psi.shape[0]
+        mats, _ = prog.tables(torch.empty((B, 0), dtype=torch.float32, device=psi.device), False)
+        prog.forward_passes(out, mats)
+        return out
 
     def post_measurement_states(self, psi_row, qubits):
         """Normalised post-measurement states of one circuit for every outcome of ``qubits`` (caller's
@@ -440,7 +521,17 @@ class SVExecutor:
         for spec, g in zip(self.specs, grads):
             if g is None or spec.kind == "zexp":
                 continue
-            if spec.kind == "fexp":
+            if spec.kind == "fexp" and spec.basis is not None:
+                # lam += g O psi with O = (x V) [sum_S c_S Z_S] (x V^dagger)
+                phi = self._basis_apply(spec, psi, inverse=True)
+                coef = g.to(torch.float64).reshape(B, 1) * torch.tensor(spec.bcoefs, dtype=torch.float64,
+                                                                        device=dev)
+                mu = torch.empty_like(psi)
+                hip.call("tedq_sv_lambda_z", hip.ptr(phi), hip.ptr(mu), B, self.n, hip.ptr(self._basis_masks(spec, dev)),
+                         len(spec.bmasks), hip.ptr(coef.contiguous()), st)
+                mu = self._basis_apply(spec, mu, inverse=False, in_place=True)
+                hip.call("tedq_sv_axpy", hip.ptr(lam), hip.ptr(mu), B, self.n, 1.0, st)
+            elif spec.kind == "fexp":
                 bits, fac = _factored_args(spec, dev)
                 coef = g.to(torch.float64).reshape(B).contiguous()
                 hip.call("tedq_sv_lambda_factored", hip.ptr(psi), hip.ptr(lam), B, self.n, len(spec.fbits),

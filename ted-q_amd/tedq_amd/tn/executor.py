@@ -37,12 +37,30 @@ import torch
 from ..engine import hip
 from ..engine.sv import MeasSpec, check_device_flag
 from ..measurement import Expectation, Probability, State
-from .network import cancel_inverse_pairs
+from .network import cancel_inverse_pairs, diagonal_reduce
 from .path import external_path, path_sets, path_stats, search_path
 from .slicing import choose_slices, slice_network
 
 _BLOCK_CODE = {"RX": 14, "RY": 15, "RZ": 16, "Rot": 17, "PhaseShift": 18,
                "CRX": 14, "CRY": 15, "CRZ": 16, "ControlledPhaseShift": 18}
+# gates that are diagonal for every parameter value (their tensors become hyper-index diagonals)
+_DIAGONAL_GATES = {"RZ", "PhaseShift", "PauliZ", "S", "T", "I", "CZ", "CRZ", "ControlledPhaseShift"}
+
+
+def _is_diagonal_matrix(m):
+    m = np.asarray(m)
+    return m.ndim == 2 and m.shape[0] == m.shape[1] and not np.any(m - np.diag(np.diag(m)))
+
+
+def diagonal_gate(op):
+    """True when the operator's matrix is diagonal whatever its parameters (by type), or it is a
+    constant diagonal matrix."""
+    name = op.name[:-4] if getattr(op, "is_inverse", False) else op.name
+    if name in _DIAGONAL_GATES:
+        return True
+    if op.num_params == 0:
+        return _is_diagonal_matrix(np.asarray(op.matrix).reshape(2 ** len(op.qubits), -1))
+    return False
 _NO_SRC = -2 ** 31
 _DESC_WORDS = 4 + 9 * 32
 _ROLE = {"AH": 0, "AM": 1, "AK": 2, "BH": 3, "BK": 4, "BN": 5, "CH": 6, "CM": 7, "CN": 8}
@@ -138,6 +156,8 @@ def _step_layout(la, lb, res):
     (wire i of a gate's [new..., old...] pairs new[i] with old[i]), so the step reads and writes the large
     tensor at the same addresses; otherwise A's kept indices, then B's."""
     big, small = (la, lb) if len(la) >= len(lb) else (lb, la)
+    if len(small) <= 8 and set(res) == set(big):
+        return list(big)          # a diagonal (hyper-index) factor: elementwise, in the big tensor's layout
     if len(small) <= 8:
         sbig = set(big)
         summed = [x for x in small if x in sbig and x not in res]
@@ -161,14 +181,20 @@ class _Step:
 class _Network:
     """One reduced network with its frozen path and index layouts."""
 
-    def __init__(self, tn, leaf_src, desc_rows, trials, external=None, slicing=None, sliced=None):
+    def __init__(self, tn, leaf_src, desc_rows, trials, external=None, slicing=None, sliced=None, is_diagonal=None,
+                 searched=None):
         # an externally found order applies to the network as built (no light-cone reduction)
         red = tn if external is not None else cancel_inverse_pairs(tn)
+        diag = [False] * red.num_tensors
+        if is_diagonal is not None and external is None:
+            red, diag = diagonal_reduce(red, is_diagonal)
+        self.n_diagonal = sum(diag)
         inputs = [list(t) for t in red.input_indices]
         sources = list(red.sources)
         if len(inputs) < 2:                       # keep at least one step
             inputs.append([])
             sources.append(("one",))
+            diag.append(False)
             if external is not None:
                 external = [(0, 1)]
         self.measurement = tn.measurement
@@ -179,6 +205,8 @@ class _Network:
             largest, _ = path_stats(path, sets, len(inputs))
             info = {"strategy": "external", "trial": None,
                     "macs": sum(1 << len(sets[i] | sets[j]) for i, j in path), "largest_rank": largest}
+        elif searched is not None:
+            path, sets, info = searched
         else:
             path, sets, info = search_path(inputs, self.output, trials)
         # sliced indices: given with an external tree, or chosen on the frozen path from slicing options
@@ -190,13 +218,14 @@ class _Network:
         self.sel_off = -1          # first constant slot of the selector vectors (set by the executor)
         if sliced:
             inputs, sources, path = slice_network(inputs, sources, self.output, path, sliced)
+            diag += [False] * (len(inputs) - len(diag))
             sets = path_sets(inputs, self.output, path)
             largest, _ = path_stats(path, sets, len(inputs))
             info = dict(info, sliced=list(sliced), slices=1 << len(sliced), largest_rank=largest,
                         macs=sum(1 << len(sets[i] | sets[j]) for i, j in path) << len(sliced))
         self.n_inputs = len(inputs)
         self.sources = sources
-        self.leaves = [leaf_src(s) for s in sources]
+        self.leaves = [leaf_src(s, d) for s, d in zip(sources, diag)]
         self.path_info = info
         self.path = path
         lay = [list(t) for t in inputs]
@@ -294,7 +323,8 @@ class TNExecutor:
     ``specs`` and ``n_params``.
     """
 
-    def __init__(self, cc, trials=None, max_bytes=None, paths=None, gemm_form="4m", slicing=None):
+    def __init__(self, cc, trials=None, max_bytes=None, paths=None, gemm_form="4m", slicing=None,
+                 diagonal_reduction=None):
         from .path import SEARCH_TRIALS  # pylint: disable=import-outside-toplevel
         trials = SEARCH_TRIALS if trials is None else trials
         self.n = cc._num_qubits  # pylint: disable=protected-access
@@ -306,6 +336,20 @@ class TNExecutor:
         # backward: both environments of a gate step from one pass (tedq_tn_gate_backward)
         self.fused_backward = True
         self._small = {}
+        ext = _external_paths(cc, paths)
+        # diagonal reduction (the reference's tn_simplify, by gate type): diagonal gates are stored as their
+        # diagonals and their wires merged into hyper-indices.  Off for externally given orders, which
+        # refer to the network as built; otherwise used when the modelled device time of the searched
+        # paths (summed over the networks) is lower with it than without it
+        self._diag_ops = {idx: diagonal_gate(op) for idx, op in cc._operators.items()}  # pylint: disable=protected-access
+        self._searched = None
+        self.simplify = False
+        self.simplify_report = None
+        allowed = bool(getattr(cc, "_tn_simplify", True)) and all(p is None for p, _ in ext)
+        if diagonal_reduction is not None:          # forced on or off (tests, experiments)
+            self.simplify = allowed and bool(diagonal_reduction)
+        elif allowed and any(self._diag_ops.values()):
+            self.simplify, self._searched, self.simplify_report = self._choose_reduction(cc, trials)
         pool = []          # complex128 constants: matrices and constant parameter values
         recs = []
         self._gate_off = {}
@@ -337,15 +381,17 @@ class TNExecutor:
                 pool.extend(mat.reshape(-1).tolist())
                 code, kind = 0, 1
             dag = int(bool(getattr(op, "is_inverse", False)))
-            ket, bra = off, off + D * D
-            off += 2 * D * D
+            diag = self.simplify and self._diag_ops[idx]
+            size = D if diag else D * D          # a diagonal leaf stores its D diagonal entries
+            ket, bra = off, off + size
+            off += 2 * size
             self._gate_off[idx] = (ket, bra)
             for is_bra, o in ((0, ket), (1, bra)):
                 d = dsl if not is_bra else [-1, -1, -1]
                 recs.append([code, srcs[0], srcs[1], srcs[2], coff, nq, kind, is_bra, o, d[0], d[1], d[2],
-                             dag, 0, 0, 0])
+                             dag, int(diag), 0, 0])
             for i, _ in enumerate(trainable):
-                param_recs.append([ket, bra, nq, flat + i])
+                param_recs.append([ket, bra, nq | (16 if diag else 0), flat + i])
             flat += len(trainable)
         self.n_params = flat
         self.leaf_stride = max(off, 1)
@@ -370,7 +416,6 @@ class TNExecutor:
 
         self.networks = []
         self.specs = []
-        ext = _external_paths(cc, paths)
         # own slicing: explicit options, else the reference's hyper_opt["slicing_opts"] (cotengra keys
         # target_size = largest intermediate in elements, target_slices = least number of slices)
         if slicing is None and isinstance(cc._hyper_opt, dict):  # pylint: disable=protected-access
@@ -380,7 +425,7 @@ class TNExecutor:
 
             selectors = {}
 
-            def leaf_src(s, meas=meas, selectors=selectors):
+            def leaf_src(s, diag=False, meas=meas, selectors=selectors):
                 kind = s[0]
                 if kind == "select":
                     # selector vector of a sliced index (both caps of index j share one slot; the
@@ -400,7 +445,8 @@ class TNExecutor:
                 if kind == "one":
                     return ("const", 2)
                 mat = meas.obs[s[1]].matrix if kind == "obs" else meas.obs.matrix
-                m = np.asarray(mat, dtype=np.complex64).reshape(-1)
+                m = np.asarray(mat, dtype=np.complex64)
+                m = np.diag(m.reshape(int(np.sqrt(m.size)), -1)).copy() if diag else m.reshape(-1)
                 o = self._const_len
                 consts.append(m)
                 const_off.append(o)
@@ -408,7 +454,10 @@ class TNExecutor:
                 return ("const", o)
 
             ext_path, ext_sliced = ext[len(self.networks)]
-            net = _Network(tn, leaf_src, desc_rows, trials, ext_path, slicing, ext_sliced)
+
+            searched = self._searched[len(self.networks)] if self._searched else None
+            net = _Network(tn, leaf_src, desc_rows, trials, ext_path, slicing, ext_sliced,
+                           self._is_diagonal(meas) if self.simplify else None, searched)
             if net.sliced:
                 offs = [selectors[j] for j in range(len(net.sliced))]
                 if offs != list(range(offs[0], offs[0] + 2 * len(offs), 2)):
@@ -442,6 +491,38 @@ class TNExecutor:
         self._sel = {}
         self._descs = np.stack(descs).astype(np.int32) if descs else np.zeros((1, _DESC_WORDS), np.int32)
         self._dev = {}
+
+    def _is_diagonal(self, meas):
+        """Predicate on network sources: stored as a diagonal (gates by type, observables by matrix)."""
+        def pred(s):
+            if s[0] in ("gate", "adj"):
+                return self._diag_ops[s[1]]
+            if s[0] == "obs":
+                return _is_diagonal_matrix(meas.obs[s[1]].matrix)
+            if s[0] == "obsmat":
+                m = np.asarray(meas.obs.matrix)
+                return _is_diagonal_matrix(m.reshape(int(np.sqrt(m.size)), -1))
+            return False
+        return pred
+
+    def _choose_reduction(self, cc, trials):
+        """Search every (light-cone reduced) network with and without diagonal reduction; keep the variant
+        whose paths have the lower modelled device time in total.  Returns (use_reduction, the kept
+        variant's searched paths per network, report)."""
+        variants = {False: [], True: []}
+        for tn in cc._tensor_networks:  # pylint: disable=protected-access
+            red = cancel_inverse_pairs(tn)
+            for use in (False, True):
+                net = diagonal_reduce(red, self._is_diagonal(tn.measurement))[0] if use else red
+                inputs = [list(t) for t in net.input_indices]
+                if len(inputs) < 2:
+                    variants[use].append(None)
+                    continue
+                variants[use].append(search_path(inputs, list(tn.output_indices), trials))
+        model = {use: sum(v[2]["model_s"] for v in vs if v is not None) for use, vs in variants.items()}
+        use = model[True] < model[False]
+        report = {"diagonal_reduction": use, "model_s_with": model[True], "model_s_without": model[False]}
+        return use, variants[use], report
 
     # ------------------------------------------------------------------
     def _buffers(self, device):

@@ -169,3 +169,56 @@ def cancel_inverse_pairs(tn):
     for x in tn.output_indices:
         size[x] = 2
     return TensorNetwork(fin_ix, list(tn.output_indices), size, fin_src, tn.measurement)
+
+
+def diagonal_reduce(tn, is_diagonal):
+    """Collapse the wires through diagonal tensors into hyper-indices.
+
+    A tensor that is diagonal as a matrix (RZ, PhaseShift, Z, S, T, CZ, CRZ,
+    controlled phase, diagonal observables and user matrices, and their bra
+    copies) maps each wire to itself, so its output index new_i and input
+    index old_i carry the same value: they are merged into one index and the
+    tensor keeps only its diagonal, [new..., old...] -> [w...].  Chains of
+    diagonal gates on a wire become one index shared by all of them.  The
+    reference's TensorNetwork.diagonal_reduce (tensor_network.py:316-392)
+    decides this from the trace-time array values, which changes meaning when
+    a parameter later takes another value; here ``is_diagonal(source)``
+    decides it from the gate type (diagonal for every parameter value) or from
+    a constant matrix.  Output indices keep their names.  Returns
+    (network, diag) with diag[t] True for the tensors stored as diagonals.
+    """
+    parent = {}
+
+    def root(s):
+        while parent.get(s, s) != s:
+            s = parent[s]
+        return s
+
+    out = set(tn.output_indices)
+    diag = []
+    for t, (ix, s) in enumerate(zip(tn.input_indices, tn.sources)):
+        d = s[0] in ("gate", "adj", "obs", "obsmat") and len(ix) % 2 == 0 and len(ix) > 0 and is_diagonal(s)
+        diag.append(d)
+        if not d:
+            continue
+        k = len(ix) // 2
+        for a, b in zip(ix[:k], ix[k:]):
+            ra, rb = root(a), root(b)
+            if ra == rb:
+                continue
+            # the merged wire is named by an output index when it holds one, else by its earliest symbol
+            if ra in out and rb in out:
+                raise ValueError("diagonal reduction would merge two output indices")
+            keep, drop = (ra, rb) if ra in out else (rb, ra)
+            parent[drop] = keep
+    new_ix = []
+    for ix, d in zip(tn.input_indices, diag):
+        if d:
+            k = len(ix) // 2
+            new_ix.append([root(x) for x in ix[k:]])
+        else:
+            new_ix.append([root(x) for x in ix])
+    size = {x: 2 for t in new_ix for x in t}
+    for x in tn.output_indices:
+        size[x] = 2
+    return TensorNetwork(new_ix, list(tn.output_indices), size, list(tn.sources), tn.measurement), diag

@@ -9,11 +9,12 @@ memory drops by up to 2^s at the price of repeated work.
 
 Here a slice is expressed inside the network itself, so that every slice
 runs the same frozen path, layouts and step descriptors: a bond index ``s``
-shared by tensors T1 and T2 is renamed ``s'`` in T2, and two selector
-vectors e_v (|0> or |1>, one on ``s`` and one on ``s'``) are appended as
-leaves and contracted into T1 and T2 before the original path continues.
-Summing over v the product e_v(s) e_v(s') is the identity on the bond, so
-the sum over all assignments equals the unsliced network.  The selector
+shared by tensors T1 ... Tm (m = 2 for a plain bond, more for a hyper-index
+left by diagonal reduction) is renamed in T2 ... Tm, and m selector vectors
+e_v (|0> or |1>, one on each copy) are appended as leaves and contracted into
+their tensors before the original path continues.  Summing over v the
+product of the selectors gives back the bond (the delta tensor of the
+hyper-index), so the sum over all assignments equals the unsliced network.  The selector
 vectors live in the executor's constant arena; between slices only their
 values change.
 
@@ -51,7 +52,7 @@ def choose_slices(path, sets, n_inputs, output, target_size=None, target_slices=
     for t in sets[:n_inputs]:
         for ix in t:
             counts[ix] = counts.get(ix, 0) + 1
-    bonds = {ix for ix, c in counts.items() if c == 2 and ix not in out}
+    bonds = {ix for ix, c in counts.items() if c >= 2 and ix not in out}
     sliced = set()
     while len(sliced) < max_sliced:
         cur = _widest(inter, sliced)
@@ -93,16 +94,19 @@ def slice_network(inputs, sources, output, path, sliced):
     caps = []           # (tensor it caps, index symbol, slice number)
     for j, s in enumerate(sliced):
         hs = holders.get(s, [])
-        if s in out or len(hs) != 2 or hs[0] == hs[1]:
-            raise ValueError(f"index {s!r} is not a bond between two tensors and cannot be sliced")
-        while get_symbol(nxt) in used:
-            nxt += 1
-        s2 = get_symbol(nxt)
-        used.add(s2)
-        t1, t2 = hs
-        new_inputs[t2] = [s2 if x == s else x for x in new_inputs[t2]]
-        caps.append((t1, s, j))
-        caps.append((t2, s2, j))
+        if s in out or len(hs) < 2 or len(set(hs)) != len(hs):
+            raise ValueError(f"index {s!r} is not a bond between tensors and cannot be sliced")
+        # every holder but the first gets its own copy of the index; a selector on each copy (the same
+        # vector for all of them) fixes the value, summing the products over v gives back the bond (or
+        # the hyper-index's delta)
+        caps.append((hs[0], s, j))
+        for t in hs[1:]:
+            while get_symbol(nxt) in used:
+                nxt += 1
+            s2 = get_symbol(nxt)
+            used.add(s2)
+            new_inputs[t] = [s2 if x == s else x for x in new_inputs[t]]
+            caps.append((t, s2, j))
     n0 = len(inputs)
     new_sources = list(sources)
     for _, sym, j in caps:

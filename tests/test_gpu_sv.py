@@ -580,3 +580,60 @@ def test_more_than_32_z_string_outputs():
     e_g = relerr(t.grad.cpu().numpy(), t64.grad.numpy(), GFLOOR)
     print(f"40 Z-string outputs: values relerr {e_v:.2e}, gradients relerr {e_g:.2e}", flush=True)
     assert e_v < TOL and e_g < TOL
+
+
+def test_product_observable_with_many_non_z_factors():
+    """expval of a product of more than 4 non-Z single-qubit factors (the reference multiplies any number,
+    pytorch_backend.py:402-424): X, Y, Hadamard and a Hermitian matrix with unequal eigenvalues on six
+    qubits plus a Z; each factor is diagonalised, the state rotated into their eigenbases by a pass
+    program and the value summed over the Z-strings of the expansion in fp64.  Values and gradients
+    against the oracle."""
+    n = 8
+    rng = np.random.default_rng(44)
+    spec = _random_spec(n, 40, rng)
+    npar = sum(s[2] for s in spec)
+    herm = np.array([[0.7, 0.2 - 0.4j], [0.2 + 0.4j, -0.1]])
+
+    def meas():
+        return qai.expval([qai.PauliX(qubits=[0]), qai.PauliY(qubits=[1]), qai.Hadamard(qubits=[2]),
+                           qai.Unitary(herm, qubits=[3]), qai.PauliX(qubits=[4]), qai.PauliY(qubits=[5]),
+                           qai.PauliZ(qubits=[7])])
+    c, _ = _build(n, spec, meas)
+    ex = c.executor()
+    assert ex.specs[0].basis is not None and len(ex.specs[0].bmasks) == 2
+    vals = rng.uniform(-math.pi, math.pi, npar)
+    t = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = c(t)
+    out.sum().backward()
+    t64 = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    obs = [("PauliX", [0]), ("PauliY", [1]), ("Hadamard", [2]), (herm, [3]), ("PauliX", [4]), ("PauliY", [5]),
+           ("PauliZ", [7])]
+    ref = O.execute(n, _oracle_gates(spec, list(t64)), [("expval", obs)])
+    ref.sum().backward()
+    e_v = abs(float(out[0]) - float(ref[0]))
+    e_g = relerr(t.grad.cpu().numpy(), t64.grad.numpy(), GFLOOR)
+    print(f"7-factor product observable: value abs err {e_v:.2e}, gradients relerr {e_g:.2e}", flush=True)
+    assert e_v < 1e-5 and e_g < TOL
+
+
+def test_specialised_adjoint_plan_on_the_interpreted_kernels():
+    """The adjoint plan with unfused rotations (real 2x2 ops, single-value gradients in groups of 8) runs
+    on the library's interpreted pass kernel as well as on generated ones: gradients against the oracle
+    (1e-5 relative) on a random circuit with every rotation kind, controlled and inverted."""
+    n = 11
+    rng = np.random.default_rng(61)
+    spec = _random_spec(n, 60, rng)
+    obs = [0, 5, 9]
+    c, npar = _build(n, spec, lambda: [qai.expval(qai.PauliZ(qubits=[q])) for q in obs], tile_bits=8, reg_bits=3)
+    ex = c.executor()
+    ex.bwd = ex._program(ex.bwd.plan.k, ex.bwd.plan.r, adjoint=True, specialize=True)  # pylint: disable=protected-access
+    assert any(sl.scomp >= 0 for sl in ex.bwd.plan.slots)
+    vals = rng.uniform(-math.pi, math.pi, size=(2, npar))
+    rows = torch.tensor(vals, dtype=torch.float32, device=DEV)
+    outs, grad = ex.value_and_grad(rows, [torch.ones(2, device=DEV) for _ in obs])
+    t64 = torch.tensor(vals[1], dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, _oracle_gates(spec, list(t64)), [("expval", ("PauliZ", [q])) for q in obs])
+    ref.sum().backward()
+    e_g = relerr(grad[1].cpu().numpy(), t64.grad.numpy(), GFLOOR)
+    print(f"specialised adjoint on interpreted kernels: gradients relerr {e_g:.2e}", flush=True)
+    assert e_g < TOL

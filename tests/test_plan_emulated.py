@@ -225,3 +225,31 @@ def test_specialized_adjoint_with_inverted_and_controlled_rotations():
         gates.append((name, qubits, [-x for x in ps] if iv else ps))
     O.execute(n, gates, [("expval", ("PauliZ", [1]))]).sum().backward()
     assert np.allclose(grad, t.grad.numpy(), atol=1e-9)
+
+
+def test_basis_expansion_of_many_factor_observables():
+    """A product of more than 4 Hermitian single-qubit factors is rewritten as (x V) [sum_S c_S Z_S]
+    (x V^dagger); the rewrite reproduces the dense product operator (numpy, 6 qubits)."""
+    from tedq_amd.engine.sv import _basis_spec
+    rng = np.random.default_rng(3)
+    n = 6
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    Y = np.array([[0, -1j], [1j, 0]])
+    H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    A = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    herm = A + A.conj().T
+    rest = {0: X, 1: Y, 2: H, 3: herm, 5: herm @ herm}
+    zmask = 1 << 4
+    spec = _basis_spec(zmask, rest)
+    assert len(spec.bmasks) == 2 ** 2      # herm and herm^2 are not traceless
+
+    def embed(ops):
+        m = np.array([[1.0 + 0j]])
+        for bit in reversed(range(n)):        # physical bit n-1 is the most significant
+            m = np.kron(m, ops.get(bit, np.eye(2)))
+        return m
+    Z = np.diag([1.0, -1.0]).astype(complex)
+    dense = embed({**rest, 4: Z})
+    V = embed({b: v for b, v in spec.basis})
+    zsum = sum(c * embed({b: Z for b in range(n) if (m >> b) & 1}) for m, c in zip(spec.bmasks, spec.bcoefs))
+    assert np.allclose(V @ zsum @ V.conj().T, dense, atol=1e-12)

@@ -236,7 +236,9 @@ def test_fused_gate_backward_dispatch_keeps_gradients():
     ref = O.execute(n, gates, [("expval", ("PauliZ", [obs]))])
     ref.sum().backward()
     for fused in (True, False):
-        cc = qai.Circuit(cd, n, *[torch.tensor(t) for t in theta]).compilecircuit(backend="pytorch", tn_mode=True)
+        # without diagonal reduction, so that RZ / CZ stay gates and the large steps are plain gate steps
+        cc = qai.Circuit(cd, n, *[torch.tensor(t) for t in theta]).compilecircuit(backend="pytorch", tn_mode=True,
+                                                                                  tn_simplify=False)
         ex = install(TNExecutor(cc))
         ex.fused_backward = fused
         calls = []
@@ -417,8 +419,8 @@ def test_sliced_contraction_equals_the_unsliced_one_and_the_oracle():
     """slicing_opts (the reference's hyper_opt key): the engine slices bond indices on its frozen path
     until every intermediate fits target_size, runs the sliced network once per slice (selector vectors
     in the constant arena) and adds values and gradients; they equal the unsliced run and the oracle.
-    On the 3x3 lattice of 2 layers <Z_4> has rank-11 intermediates; a target of 2^6 elements slices
-    six indices (64 slices)."""
+    On the 3x3 lattice of 2 layers the <Z_4> network's intermediates exceed 2^6 elements; reaching that
+    target takes several sliced (hyper-)indices."""
     from tedq_amd import workloads as W
     L, layers, obs = 3, 2, [4, 0]
     n = L * L
@@ -426,7 +428,7 @@ def test_sliced_contraction_equals_the_unsliced_one_and_the_oracle():
     sl_ex = install(W.lattice(L, layers, observe=obs, tn_mode=True,
                               hyper_opt={"slicing_opts": {"target_size": 2 ** 6}}).executor())
     rep_full, rep_sl = full.path_report(), sl_ex.path_report()
-    assert len(sl_ex.networks[0].sliced) >= 4
+    assert len(sl_ex.networks[0].sliced) >= 2
     assert max(r["largest_rank"] for r in rep_sl) <= 6 < max(r["largest_rank"] for r in rep_full)
     rng = np.random.default_rng(11)
     theta = rng.uniform(0, 2 * np.pi, size=(2, 2 * n * layers))
@@ -495,3 +497,81 @@ def test_choose_slices_reaches_its_targets():
 def path_sets_of(net):
     from tedq_amd.tn.path import path_sets
     return path_sets([list(t) for t in net.lay[:net.n_inputs]], net.output, net.path)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_diagonal_reduction_keeps_values_and_gradients(seed):
+    """tn_simplify: diagonal gates (RZ, PhaseShift, CRZ, controlled phase, CZ, Z, S, T, a constant diagonal
+    Unitary, diagonal observables) are stored as diagonals and their wires merged into hyper-indices;
+    values and gradients equal the unsimplified network's and the oracle's, for expectation values and
+    probabilities (output indices keep their names)."""
+    rng = np.random.default_rng(100 + seed)
+    n = 5
+    diag_u = np.diag(np.exp(1j * rng.uniform(0, 6, 4)))
+    names = ["RX", "RZ", "PhaseShift", "CRZ", "ControlledPhaseShift", "CZ", "PauliZ", "S", "T", "Hadamard",
+             "CNOT", "RY", "U2d"]
+    npar = {"RX": 1, "RZ": 1, "PhaseShift": 1, "CRZ": 1, "ControlledPhaseShift": 1, "RY": 1}
+    spec = []
+    for _ in range(24):
+        name = names[rng.integers(len(names))]
+        k = 2 if name in ("CRZ", "ControlledPhaseShift", "CZ", "CNOT", "U2d") else 1
+        spec.append((name, [int(q) for q in rng.choice(n, size=k, replace=False)]))
+    P = sum(npar.get(nm, 0) for nm, _ in spec)
+    q0, q1 = [int(x) for x in rng.choice(n, size=2, replace=False)]
+    kind = seed % 2
+
+    def cd(*p):
+        it = iter(p)
+        for name, qs in spec:
+            if name == "U2d":
+                qai.Unitary(diag_u, qubits=qs)
+            else:
+                getattr(qai, name)(*[next(it) for _ in range(npar.get(name, 0))], qubits=qs)
+        if kind == 0:
+            return [qai.expval(qai.PauliZ(qubits=[q0])), qai.expval([qai.PauliZ(qubits=[q0]), qai.PauliX(qubits=[q1])])]
+        return [qai.probs(qubits=sorted([q0, q1]))]
+
+    theta = rng.uniform(0, 2 * np.pi, size=max(P, 1))[:P]
+    rows = torch.tensor(np.stack([theta, theta[::-1].copy()]), dtype=torch.float32)
+    res = {}
+    for simplify in (True, False):
+        cc = qai.Circuit(cd, n, *[torch.tensor(t) for t in theta]).compilecircuit(
+            backend="pytorch", tn_mode=True, tn_simplify=simplify)
+        ex = install(TNExecutor(cc, diagonal_reduction=simplify))
+        if simplify:
+            assert sum(net.n_diagonal for net in ex.networks) > 0
+        ups = [torch.ones(2) if s.kind == "fexp" else torch.ones(2, 4) for s in ex.specs]
+        res[simplify] = ex.value_and_grad(rows, ups)
+    (v1, g1), (v0, g0) = res[True], res[False]
+    for a, b in zip(v1, v0):
+        assert np.max(np.abs(a.numpy() - b.numpy())) < 2e-6
+    if P:
+        assert np.max(np.abs(g1.numpy() - g0.numpy())) < 2e-6
+    th = torch.tensor(rows[0].numpy(), dtype=torch.float64, requires_grad=True)
+    it = iter(th)
+    gates = []
+    for name, qs in spec:
+        if name == "U2d":
+            gates.append(("Unitary", qs, diag_u))
+        else:
+            gates.append((name, qs, [next(it) for _ in range(npar.get(name, 0))]))
+    ms = ([("expval", ("PauliZ", [q0])), ("expval", [("PauliZ", [q0]), ("PauliX", [q1])])] if kind == 0
+          else [("probs", sorted([q0, q1]))])
+    out = O.measure(O.propagate(n, gates), n, ms)
+    for o, r in zip(v1, out):
+        assert np.max(np.abs(o[0].numpy().reshape(-1) - r.detach().numpy().reshape(-1))) < 2e-5
+    if P:
+        (gref,) = torch.autograd.grad(sum(r.sum() for r in out), th)
+        assert np.max(np.abs(g1[0].numpy() - gref.numpy())) < 1e-4
+
+
+def test_diagonal_reduction_is_chosen_by_the_device_model():
+    """With tn_simplify (default) the executor searches paths with and without diagonal reduction and
+    keeps the variant of lower modelled device time; tn_simplify=False never reduces."""
+    from tedq_amd import workloads as W
+    ex = W.disordered_chain(16, 3, 8, tn_mode=True).executor()
+    rep = ex.simplify_report
+    assert rep is not None and ex.simplify == (rep["model_s_with"] < rep["model_s_without"])
+    assert (sum(net.n_diagonal for net in ex.networks) > 0) == ex.simplify
+    off = W.disordered_chain(16, 3, 8, tn_mode=True, tn_simplify=False).executor()
+    assert not off.simplify and off.simplify_report is None

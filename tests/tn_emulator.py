@@ -64,13 +64,16 @@ def install(ex):
         dm = np.zeros((B, max(ex.n_params, 1) * 4), dtype=np.complex64)
         pool = ex._pool  # pylint: disable=protected-access
         for rec in ex._recs:  # pylint: disable=protected-access
-            code, s0, s1, s2, coff, nq, kind, bra, off, d0, d1, d2, dag = (int(v) for v in rec[:13])
+            code, s0, s1, s2, coff, nq, kind, bra, off, d0, d1, d2, dag, diag = (int(v) for v in rec[:14])
             D = 1 << nq
             for b in range(B):
                 if kind == 1:
                     U = pool[coff:coff + D * D].reshape(D, D)
                     T = U.conj().T if bra else U
-                    lv[b, off:off + D * D] = T.reshape(-1)
+                    if diag:
+                        lv[b, off:off + D] = np.diag(T)
+                    else:
+                        lv[b, off:off + D * D] = T.reshape(-1)
                     continue
 
                 def val(s):
@@ -84,7 +87,11 @@ def install(ex):
                     V = V.conj().T
                 U = np.eye(D, dtype=np.complex128)
                 U[D - 2:, D - 2:] = V
-                lv[b, off:off + D * D] = (U.conj().T if bra else U).reshape(-1)
+                T = U.conj().T if bra else U
+                if diag:
+                    lv[b, off:off + D] = np.diag(T)
+                else:
+                    lv[b, off:off + D * D] = T.reshape(-1)
                 for w, d in enumerate((d0, d1, d2)):
                     if d < 0:
                         continue
@@ -136,11 +143,16 @@ def install(ex):
         dm = dmats.numpy()
         g = grad.numpy()
         for p, (ket, bra, nq, ds) in enumerate(ex._param_recs):  # pylint: disable=protected-access
-            D = 1 << int(nq)
+            diag = bool(int(nq) & 16)
+            D = 1 << (int(nq) & 15)
             for b in range(B):
                 dV = dm[b, ds * 4:ds * 4 + 4].astype(np.complex128).reshape(2, 2)
-                Ek = env[b, ket:ket + D * D].reshape(D, D)[D - 2:, D - 2:]
-                Eb = env[b, bra:bra + D * D].reshape(D, D)[D - 2:, D - 2:]
+                if diag:
+                    Ek = np.diag(env[b, ket + D - 2:ket + D].astype(np.complex128))
+                    Eb = np.diag(env[b, bra + D - 2:bra + D].astype(np.complex128))
+                else:
+                    Ek = env[b, ket:ket + D * D].reshape(D, D)[D - 2:, D - 2:]
+                    Eb = env[b, bra:bra + D * D].reshape(D, D)[D - 2:, D - 2:]
                 g[b, p] += float(np.real((Ek * dV).sum() + (Eb * dV.conj().T).sum()))
 
     ex._leaves = leaves  # pylint: disable=protected-access
