@@ -515,7 +515,38 @@ __device__ __forceinline__ int ragged_record(const RaggedRec* __restrict__ recs,
     if (recs[mid].tile_end > blk) hi = mid;
     else lo = mid + 1;
   }
-  return lo;
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// The record is the same for the whole workgroup: read every word through readfirstlane so that the
+// compiler keeps pointers, masks and sizes in scalar registers (as kernel arguments are).
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// A pointer read from memory is generic (flat); taking it through the global address space lets the
+// compiler issue global (and, for uniform read-only words such as the program, scalar) loads, as it does
+// for pointer kernel arguments.
+template <class T>
+__device__ __forceinline__ T* global_ptr(const void* p) {
+  return (T*)(__attribute__((address_space(1))) T*)uniform64(reinterpret_cast<uint64_t>(p));
+}
+
+__device__ __forceinline__ RaggedRec load_record(const RaggedRec* __restrict__ recs, int c) {
+  RaggedRec r;
+  const RaggedRec& g = recs[c];
+  r.psi = global_ptr<float2>(g.psi);
+  r.lam = global_ptr<float2>(g.lam);
+  r.prog = global_ptr<const int>(g.prog);
+  r.mats = global_ptr<const float2>(g.mats);
+  r.partials = global_ptr<float>(g.partials);
+  r.rest_mask = uniform64(g.rest_mask);
+  r.tile_end = (int64_t)uniform64((uint64_t)g.tile_end);
+  r.n = __builtin_amdgcn_readfirstlane(g.n);
+  r.n_gslots = __builtin_amdgcn_readfirstlane(g.n_gslots);
+  return r;
 }
 
 __device__ __forceinline__ PassArgs ragged_args(const RaggedRec& rc, int k, int tT) {
@@ -538,8 +569,8 @@ template <int R>
 __global__ void __launch_bounds__(512) k_pass_fwd_ragged(const RaggedRec* __restrict__ recs, int nrec, int k,
                                                          int tT) {
   const int c = ragged_record(recs, nrec, blockIdx.x);
-  const RaggedRec rc = recs[c];
-  const int64_t first = c ? recs[c - 1].tile_end : 0;
+  const RaggedRec rc = load_record(recs, c);
+  const int64_t first = c ? (int64_t)uniform64((uint64_t)recs[c - 1].tile_end) : 0;
   pass_fwd_body<R, false>(rc.psi, rc.prog, rc.mats, ragged_args(rc, k, tT), (int64_t)blockIdx.x - first);
 }
 
@@ -547,8 +578,8 @@ template <int R>
 __global__ void __launch_bounds__(512) k_pass_adj_ragged(const RaggedRec* __restrict__ recs, int nrec, int k,
                                                          int tT) {
   const int c = ragged_record(recs, nrec, blockIdx.x);
-  const RaggedRec rc = recs[c];
-  const int64_t first = c ? recs[c - 1].tile_end : 0;
+  const RaggedRec rc = load_record(recs, c);
+  const int64_t first = c ? (int64_t)uniform64((uint64_t)recs[c - 1].tile_end) : 0;
   pass_adj_body<R, false>(rc.psi, rc.lam, rc.prog, rc.mats, rc.partials, ragged_args(rc, k, tT),
                           (int64_t)blockIdx.x - first);
 }
