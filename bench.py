@@ -294,6 +294,10 @@ def serving_leg(n_req, warm, ragged=True):
     closed = serving_stream(n_req)
     keys = sorted({r.key for r in closed} | {r.key for r in serving_stream(n_req, seed=5, rate=1.0)})
     eng.prepare(keys, buckets=(1, 2, 4), log=progress)
+    # every request simulates the backward light cone of Z_0 only (trailing gates folded into the
+    # observable, qubits left without gates dropped; exact)
+    sim_n = float(np.mean([factory(r.key).executor().n for r in closed]))
+    wide_n = float(np.mean([r.key[0] for r in closed]))
     eng.run(serving_stream(warm, seed=99))
     eng.completed.clear()
     eng.batch_sizes.clear()
@@ -313,7 +317,7 @@ def serving_leg(n_req, warm, ragged=True):
              f"mean batch {mean_batch:.2f}")
     if not ragged:
         return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "evals_per_s": st["evals_per_s"],
-                "capacity_evals_per_s": cap["evals_per_s"], "mean_batch": mean_batch,
+                "arrival_rate": rate, "capacity_evals_per_s": cap["evals_per_s"], "mean_batch": mean_batch,
                 "closed_loop_mean_batch": closed_batch}
     single = ServingEngine(factory, hbm_budget=64 << 30, max_batch=1, n_streams=1, max_inflight=1,
                            graph_max_qubits=0)
@@ -328,7 +332,8 @@ def serving_leg(n_req, warm, ragged=True):
             "unbatched_evals_per_s": un["evals_per_s"], "graphs_captured": len(eng._graphs),
             "batching": "ragged: requests of different structures sharing a tile shape run in one launch "
                         "per pass index; identical structures also via captured graphs",
-            "stream": "Poisson at 0.7x closed-loop capacity; n~U{6..24}, layers~U{2..20}; <Z_0> + full gradient"}
+            "stream": "Poisson at 0.7x closed-loop capacity; n~U{6..24}, layers~U{2..20}; <Z_0> + full gradient",
+            "qubits_mean": wide_n, "qubits_simulated_mean": sim_n}
 
 
 def measured_traffic(kernel, n, layers, batch, ex):
@@ -553,6 +558,9 @@ def main():
                    "params_per_circuit": 2 * n * L, "tile_bits": ex.fwd.plan.k, "reg_bits": ex.fwd.plan.r,
                    "tile_bits_bwd": ex.bwd.plan.k, "reg_bits_bwd": ex.bwd.plan.r,
                    "passes": ex.n_passes, "passes_bwd": len(ex.bwd.plan.passes),
+                   # exact simplifications: the last RZ layer and CNOT chain folded into the Z-string
+                   # observables; diagonal gates applied without their global phase (expectations only)
+                   "folded_gates": ex.folded, "phase_free_forward": ex.fwd.plan.specialize,
                    "pass_kernels": "interpreted" if args.no_jit else
                    {"generated_fwd": jit_used[0], "generated_bwd": jit_used[1], "compile_s": jit_s}},
         "autotune": tuning,

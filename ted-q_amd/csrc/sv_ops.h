@@ -20,6 +20,9 @@ enum OpCode : int {
   kOpPauliGrad = 11,
   // write the pending gradient group (8 values, reduced over the tile) to slot word[5] and clear it
   kOpGradFlush = 12,
+  // amplitudes whose bit is 1 (and all control bits) times the slot's entry 1 (its conjugate with the
+  // dagger flag): a diagonal gate without its global phase (the slot table stores p = d1 conj(d0) there)
+  kOpPhase1 = 13,
 };
 
 // kinds of kOpPauliGrad: the value equals 2 Re sum_ab A_ab C_ab for A = dU U^dagger of RX (X), RY (Y),

@@ -247,6 +247,13 @@ __device__ __forceinline__ void apply_op(float2 (&v)[R], float2 (&w)[R], const i
       if (TWO) op_diag<R>(w, op[1], op[3], d0, d1);
       break;
     }
+    case kOpPhase1: {
+      const float2 q = dag ? cconj(mb[op[4] + 1]) : mb[op[4] + 1];
+      const int cm = op[3] | (1 << op[1]);
+      op_diag<R>(v, op[1], cm, q, q);
+      if (TWO) op_diag<R>(w, op[1], cm, q, q);
+      break;
+    }
     case kOpCX:
       op_cx<R>(v, op[1], op[3]);
       if (TWO) op_cx<R>(w, op[1], op[3]);

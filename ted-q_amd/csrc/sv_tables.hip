@@ -27,7 +27,9 @@ enum GateCode : int {
 // one trainable parameter gets its gradient value directly in component c of gslot (kOpPauliGrad): then
 // A is the constant that picks that component out of 2 Re sum A_ij C_ij.
 struct SlotRec {
-  int kind, ncomp, comp_start, mat_off, gslot, scalar, pad1, pad2;
+  // phase = 1: a diagonal slot diag(d0, d1) also stores p = d1 conj(d0) in its (zero) entry 1, the phase
+  // that kOpPhase1 applies to the |1> half (the gate up to its global phase d0)
+  int kind, ncomp, comp_start, mat_off, gslot, scalar, phase, pad2;
 };
 struct CompRec {
   int code, dagger, src0, src1, src2, const_off, pad0, pad1;
@@ -187,6 +189,10 @@ __global__ void k_build_tables(const PT* __restrict__ params, int64_t batch, int
     ok = ok && ok1;
   }
   for (int i = 0; i < 4; ++i) out[i] = make_float2((float)S.a[i].x, (float)S.a[i].y);
+  if (sr.phase) {
+    const cd p = cd_mul(S.a[3], cd_conj(S.a[0]));
+    out[1] = make_float2((float)p.x, (float)p.y);
+  }
   if (!ok) atomicOr(err, 1);
   if (amats == nullptr || sr.gslot < 0) return;
   if (sr.scalar > 0) {

@@ -22,7 +22,11 @@ Beyond the reference: ``execute_batch(params_2d)`` evaluates B parameter rows
 in one set of launches (shape [B, num_measurements, ...]).  The device is held
 per instance, not per class.  Every state-vector forward also reduces
 <psi|psi> in fp64 on the device (``last_norm``); ``norm_tol=<float>`` makes
-each call raise when max |<psi|psi> - 1| exceeds it.
+each call raise when max |<psi|psi> - 1| exceeds it.  When every measurement
+is a Z-string expectation, trailing gates that map Z-strings to Z-strings
+(diagonal gates, CNOTs, gates outside the observables' backward light cone)
+are folded into the observables and qubits left without gates are dropped;
+``fold=False`` keeps the whole circuit.
 """
 
 import torch
@@ -138,6 +142,9 @@ class PyTorchBackend(CompiledCircuit):
         self._tn_gemm_form = kwargs.get("tn_gemm_form", "4m")
         # state-vector mode: check max |<psi|psi> - 1| (fp64 on the device) after every forward
         self._norm_tol = kwargs.get("norm_tol", None)
+        # state-vector mode, Z-string expectations only: fold trailing gates into the observables and
+        # drop qubits left without gates (exact; False keeps every gate and qubit)
+        self._fold = bool(kwargs.get("fold", True))
         if self._interface != "pytorch":
             raise ValueError(f"{self._interface}: pytroch_backend only supports pytorch interface!")
         self._device = None
@@ -213,7 +220,7 @@ class PyTorchBackend(CompiledCircuit):
                     init = self._init_state.matrix
                 self._executor = SVExecutor(n, units, n_params, pool, self._measurements, init_state=init,
                                             k_max=self._tile_bits, r=self._reg_bits,
-                                            k_bwd=self._tile_bits_bwd, r_bwd=self._reg_bits_bwd)
+                                            k_bwd=self._tile_bits_bwd, r_bwd=self._reg_bits_bwd, fold=self._fold)
                 self._executor.jit = self._jit
                 self._executor.norm_tol = self._norm_tol
         return self._executor

@@ -21,7 +21,8 @@ import os
 import numpy as np
 
 from . import hip
-from .planner import OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1, OP_R1, OP_PGRAD, OP_GFLUSH
+from .planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1, OP_R1, OP_PGRAD, OP_GFLUSH,
+                      OP_PH1)
 from .program import HDR_TILE_BITS, HDR_HI_OFF, HDR_STAGES, STAGE_WORDS, OP_WORDS, lds_swizzle
 
 _CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))), "csrc")
@@ -114,6 +115,10 @@ def _op_lines(w, R, regs, adjoint, seg):
     elif t in (OP_D1, OP_CD1):
         for v in regs:
             out.append(f"DiagOp<{R}, {b}, {cm}>::run({v}, {_diag_args(dag, off)});")
+    elif t == OP_PH1:
+        for v in regs:
+            p = f"cconj(mb[{off + 1}])" if dag else f"mb[{off + 1}]"
+            out.append(f"DiagOp<{R}, {b}, {cm | (1 << b)}>::run({v}, {p}, {p});")
     elif t == OP_CX:
         for v in regs:
             out.append(f"CXOp<{R}, {b}, {cm}>::run({v});")

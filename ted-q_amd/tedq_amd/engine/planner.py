@@ -26,7 +26,7 @@ import numpy as np
 
 # Op codes shared with csrc/sv_ops.h (kOp* constants there).
 OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_U2, OP_GRAD1, OP_CGRAD1, OP_CD1 = 1, 2, 3, 4, 5, 6, 7, 8, 9
-OP_R1, OP_PGRAD, OP_GFLUSH = 10, 11, 12
+OP_R1, OP_PGRAD, OP_GFLUSH, OP_PH1 = 10, 11, 12, 13
 OP_DAGGER = 1 << 8
 
 # kinds of OP_PGRAD (PauliGradKind in csrc/sv_ops.h) by gate code: RX, RY, RZ, PhaseShift blocks
@@ -123,6 +123,7 @@ class Plan:
     n_params: int
     n_gslots: int
     mat_stride: int        # complex64 entries per batch row of the slot table
+    specialize: bool = False   # adjoint plan of generated kernels (see build_plan)
 
 
 def gate_to_units(n, gate, srcs, const_pool):
@@ -195,6 +196,68 @@ def two_qubit_units(m, b_hi, b_lo, const_pool):
     return units
 
 
+def _is_diagonal_unit(u):
+    if u.kind == "cz":
+        return True
+    return u.kind in ("u1", "cu1") and u.comp.const_mat < 0 and u.comp.code in _DIAGONAL_CODES
+
+
+def fold_trailing(units, zmasks):
+    """Fold trailing gates into Z-string observables (every measurement is some <psi| Z_m |psi>).
+
+    A unit that no kept unit after it shares a bit with commutes to the end of the circuit, where
+    <psi| G^dagger Z_m G |psi> takes its place.  Scanning backwards:
+      - a unit on none of the observables' bits leaves them unchanged (G^dagger Z_m G = Z_m): dropped;
+      - a diagonal unit (Z, S, T, RZ, PhaseShift, CZ, controlled diagonals) commutes with them: dropped;
+      - a CNOT (target t, control c) maps Z_t to Z_c Z_t and Z_c to itself: every mask holding bit t
+        toggles bit c, and the unit is dropped;
+      - any other unit (and a user matrix, which need not be unitary) is kept, and its bits close.
+    This is exact: values are unchanged, and the gradient of a parameter whose gates were all dropped is
+    zero.  Returns (kept units in order, new masks, number of dropped units).
+    """
+    masks = list(zmasks)
+    closed = set()
+    keep = [True] * len(units)
+    for i in range(len(units) - 1, -1, -1):
+        u = units[i]
+        bits = set(u.bits)
+        if bits & closed:
+            closed |= bits
+            continue
+        support = 0
+        for m in masks:
+            support |= m
+        known = u.comp is None or u.comp.const_mat < 0
+        if known and not any((support >> b) & 1 for b in bits):
+            keep[i] = False
+        elif _is_diagonal_unit(u):
+            keep[i] = False
+        elif u.kind == "cx" and len(u.bits) == 2:
+            t, c = u.bits
+            masks = [m ^ (1 << c) if (m >> t) & 1 else m for m in masks]
+            keep[i] = False
+        else:
+            closed |= bits
+    return [u for u, k in zip(units, keep) if k], masks, keep.count(False)
+
+
+def compact_register(units, zmasks, n):
+    """Drop qubits no unit touches from a circuit that starts in |0..0> and measures Z-strings only.
+
+    Such a qubit stays |0> and contributes a factor <0|Z|0> = 1 to every Z-string, so its bit leaves the
+    masks and the state needs only the used bits (renumbered in order, at least one).  Returns
+    (units, masks, new n)."""
+    used = sorted({b for u in units for b in u.bits}) or [0]
+    if len(used) == n:
+        return units, list(zmasks), n
+    new = {b: i for i, b in enumerate(used)}
+    out = [Unit(u.kind, tuple(new[b] for b in u.bits), u.comp, u.index) for u in units]
+    masks = []
+    for m in zmasks:
+        masks.append(sum(1 << new[b] for b in used if (m >> b) & 1))
+    return out, masks, len(used)
+
+
 def _schedule(units, order, max_bits, seed_bits, allowed_bits=None):
     """Greedy topological sweep.
 
@@ -243,7 +306,8 @@ def build_plan(n, units, n_params, const_pool, k_max=12, r=4, coalesce_bits=3, s
     gate, so that a rotation keeps its own slot: real gates then run as real 2x2 ops and the gradient
     of RX / RY / RZ / PhaseShift (and controlled forms) is one value per parameter (OP_PGRAD, 2-4
     multiply-adds per amplitude pair) instead of the slot's 8 bilinears (16 per pair), collected in
-    groups of 8 per register stage."""
+    groups of 8 per register stage.  Its uncontrolled diagonal gates run as phases on the |1> half
+    (OP_PH1, one complex multiply per pair instead of two; see program.py)."""
     k = min(n, k_max)
     widest = max((len(u.bits) for u in units), default=1)
     if widest > k:
@@ -300,7 +364,7 @@ def build_plan(n, units, n_params, const_pool, k_max=12, r=4, coalesce_bits=3, s
                 gslot += 1
     return Plan(n=n, k=k, r=r, passes=passes, slots=slots,
                 const_pool=np.asarray(const_pool, dtype=np.complex128),
-                n_params=n_params, n_gslots=gslot, mat_stride=max(mat_off, 1))
+                n_params=n_params, n_gslots=gslot, mat_stride=max(mat_off, 1), specialize=specialize)
 
 
 def _build_pass(units, pass_units, tile_bits, r, slots, specialize=False):

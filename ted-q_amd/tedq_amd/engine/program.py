@@ -10,7 +10,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from .planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_GRAD1, OP_CGRAD1, OP_CD1, OP_R1, OP_PGRAD,
-                      OP_GFLUSH, OP_DAGGER, SLOT_1Q)
+                      OP_GFLUSH, OP_PH1, OP_DAGGER, SLOT_1Q)
 
 HDR_TILE_BITS = 4
 HDR_HI_OFF = 36
@@ -48,9 +48,20 @@ def _pdep(v, positions):
     return out
 
 
-def _fwd_op_words(op, slots):
+def _phase_slot(sl, specialize):
+    """Diagonal 1-qubit slot of a specialised plan: its table row also holds p = d1 conj(d0) in the
+    unused entry 1 (U = d0 diag(1, p) for U = diag(d0, d1)), read by OP_PH1."""
+    return specialize and sl.kind == SLOT_1Q and sl.diagonal
+
+
+def _fwd_op_words(op, slots, specialize=False):
+    """Forward op words.  In a specialised plan an uncontrolled diagonal U = d0 diag(1, p) is applied
+    without its global phase d0 (OP_PH1: the |1> half times p); expectation values do not see a global
+    phase, and the adjoint sweep, which applies (diag(1, p))^dagger, undoes exactly this map."""
     kind, b0, b1, cmask, si = op
     mat = slots[si].mat_off if si >= 0 else 0
+    if kind == "u1" and cmask == 0 and _phase_slot(slots[si], specialize):
+        return [OP_PH1, b0, 0, 0, mat, -1, 0, 0]
     if kind in ("u1", "cu1"):
         sl = slots[si]
         if sl.diagonal:
@@ -68,8 +79,12 @@ def _fwd_op_words(op, slots):
     return [t, b0, b1, cmask, mat, -1, 0, 0]
 
 
-def _adj_op_words(op, slots):
-    """Reverse-sweep ops for one forward op: gradient bilinears first, then U^dagger."""
+def _adj_op_words(op, slots, specialize=False):
+    """Reverse-sweep ops for one forward op: gradient bilinears first, then U^dagger.
+
+    In a specialised plan an uncontrolled diagonal U^dagger = conj(d0) diag(1, conj(p)) is applied
+    without its global phase conj(d0) (OP_PH1 with the dagger flag).  psi and lambda receive the same
+    phase, so every bilinear conj(lambda_a) psi_b, hence every gradient, is unchanged."""
     kind, b0, b1, cmask, si = op
     words = []
     if si >= 0 and slots[si].gslot >= 0:
@@ -80,13 +95,13 @@ def _adj_op_words(op, slots):
         else:
             g = OP_GRAD1 if kind == "u1" else OP_CGRAD1
             words.append([g, b0, 0, cmask, 0, sl.gslot, 0, 0])
-    w = _fwd_op_words(op, slots)
+    w = _fwd_op_words(op, slots, specialize)
     w[0] |= OP_DAGGER
     words.append(w)
     return words
 
 
-def _encode_pass(p, k, r, slots, adjoint):
+def _encode_pass(p, k, r, slots, adjoint, specialize=False):
     nst = len(p.stages)
     stages = list(reversed(p.stages)) if adjoint else p.stages
     header = np.zeros(HDR_STAGES + STAGE_WORDS * nst, dtype=np.int64)
@@ -106,14 +121,14 @@ def _encode_pass(p, k, r, slots, adjoint):
         pending = -1          # gradient group collecting OP_PGRAD values
         for op in seq:
             if adjoint:
-                for w in _adj_op_words(op, slots):
+                for w in _adj_op_words(op, slots, specialize):
                     if w[0] == OP_PGRAD and w[5] != pending:
                         if pending >= 0:
                             ops.append([OP_GFLUSH, 0, 0, 0, 0, pending, 0, 0])
                         pending = w[5]
                     ops.append(w)
             else:
-                ops.append(_fwd_op_words(op, slots))
+                ops.append(_fwd_op_words(op, slots, specialize))
         if pending >= 0:
             ops.append([OP_GFLUSH, 0, 0, 0, 0, pending, 0, 0])
         regmask = sum(1 << b for b in st.reg_bits)
@@ -138,13 +153,13 @@ def encode_plan(plan):
     chunks, fwd_off, adj_off, masks = [], [], [], []
     pos = 0
     for p in plan.passes:
-        w = _encode_pass(p, plan.k, plan.r, plan.slots, adjoint=False)
+        w = _encode_pass(p, plan.k, plan.r, plan.slots, adjoint=False, specialize=plan.specialize)
         fwd_off.append(pos)
         chunks.append(w)
         pos += len(w)
         masks.append(sum(1 << b for b in p.tile_bits))
     for p in reversed(plan.passes):
-        w = _encode_pass(p, plan.k, plan.r, plan.slots, adjoint=True)
+        w = _encode_pass(p, plan.k, plan.r, plan.slots, adjoint=True, specialize=plan.specialize)
         adj_off.append(pos)
         chunks.append(w)
         pos += len(w)
@@ -154,7 +169,8 @@ def encode_plan(plan):
     comp_list = []
     pgslot = np.full(max(plan.n_params, 1), -1, dtype=np.int32)
     for si, s in enumerate(plan.slots):
-        slot_recs[si] = [s.kind, len(s.comps), len(comp_list), s.mat_off, s.gslot, s.scomp + 1, 0, 0]
+        slot_recs[si] = [s.kind, len(s.comps), len(comp_list), s.mat_off, s.gslot, s.scomp + 1,
+                         int(_phase_slot(s, plan.specialize)), 0]
         for c in s.comps:
             srcs = list(c.srcs) + [NO_SRC] * (3 - len(c.srcs))
             comp_list.append([c.code, int(c.dagger), srcs[0], srcs[1], srcs[2], c.const_mat, 0, 0])

@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from . import hip
-from .planner import build_plan
+from .planner import build_plan, compact_register, fold_trailing
 from .program import encode_plan
 
 _Z = np.array([[1, 0], [0, -1]], dtype=np.complex128)
@@ -265,12 +265,27 @@ class SVExecutor:
     """
 
     def __init__(self, n, units, n_params, const_pool, measurements, init_state=None, k_max=12, r=4,
-                 k_bwd=None, r_bwd=None):
+                 k_bwd=None, r_bwd=None, fold=True):
         self.n = n
         self.n_params = n_params
-        self._units = units
         self._pool = list(const_pool)
         self.specs, self.zmasks = compile_measurements(n, measurements)
+        # when every measurement is a Z-string expectation, trailing gates that map Z-strings to Z-strings
+        # are folded into the observables (planner.fold_trailing) and, from |0..0>, qubits that no gate
+        # is left on are dropped (planner.compact_register): psi is then the state of the n kept qubits
+        # before the folded gates; n_circuit stays the circuit's width
+        self.n_circuit = n
+        self.folded = 0
+        if fold and self.specs and all(s.kind == "zexp" for s in self.specs):
+            units, self.zmasks, self.folded = fold_trailing(units, self.zmasks)
+            if init_state is None:
+                units, self.zmasks, self.n = compact_register(units, self.zmasks, n)
+            for s in self.specs:
+                s.zmask = self.zmasks[s.zindex]
+        n = self.n
+        self._units = units
+        # expectation values only: the forward may drop global phases (see enable_jit)
+        self.phase_free = bool(self.specs) and all(s.kind in ("zexp", "fexp") for s in self.specs)
         # every forward measurement also reduces <psi|psi> in fp64 (the Z-string of mask 0, compensated
         # sums), appended after the circuit's own masks; the last batch's values stay in last_norm
         self._zall = list(self.zmasks) + [0]
@@ -294,19 +309,24 @@ class SVExecutor:
     def enable_jit(self, device=None, forward=True, adjoint=True):
         """Compile the forward and adjoint programs into per-pass kernels (hiprtc) for ``device``.
 
-        The adjoint switches to its specialised plan (unfused rotations with single-value gradients)
-        when generated kernels can run it: that plan has twice the ops, which only generated code
-        executes without per-op overhead.  Returns (forward_jitted, adjoint_jitted).  Compiled
-        programs are kept per (k, r) shape.
+        The adjoint switches to its specialised plan (unfused rotations with single-value gradients,
+        real 2x2 ops, diagonal gates without their global phase) when generated kernels can run it:
+        that plan has about twice the ops, which only generated code executes without per-op overhead.
+        The forward does the same when every measurement is an expectation value (which no global
+        phase changes; a returned state or probabilities keep the exact forward).  Returns
+        (forward_jitted, adjoint_jitted).  Compiled programs are kept per (k, r) shape.
         """
         from . import jit  # pylint: disable=import-outside-toplevel
         device = device or torch.device("cuda", torch.cuda.current_device())
         if forward:
+            k, r = self.fwd.plan.k, self.fwd.plan.r
+            if self.phase_free and jit.supported(self.n, k, r, False):
+                self.fwd = self._program(k, r, specialize=True)
             self.fwd.enable_jit(device, True, False)
         if adjoint:
             k, r = self.bwd.plan.k, self.bwd.plan.r
             if jit.supported(self.n, k, r, True):
-                self.bwd = self._program(k, r, adjoint=True, specialize=True)
+                self.bwd = self._program(k, r, specialize=True)
             self.bwd.enable_jit(device, False, True)
         # pylint: disable=protected-access
         return self.fwd._jit_fns(device, False) is not None, self.bwd._jit_fns(device, True) is not None
@@ -317,16 +337,16 @@ class SVExecutor:
             self.enable_jit(device)
             self._jitted.add(key)
 
-    def _program(self, k, r, adjoint=False, specialize=False):
+    def _program(self, k, r, specialize=False):
         """Fusion plan for (k, r), built once per executor (with its compiled kernels, if any).  The
-        specialised adjoint plan keeps trainable rotations unfused (single-value gradients, real 2x2
-        ops) for generated kernels."""
+        specialised plan keeps trainable rotations unfused (single-value gradients, real 2x2 ops) and
+        applies diagonal gates without their global phase, for generated kernels."""
         if not hasattr(self, "_programs"):
             self._programs = {}
-        key = (k, r, adjoint and specialize)
+        key = (k, r, specialize)
         if key not in self._programs:
             self._programs[key] = _Program(self.n, self._units, self.n_params, list(self._pool), k, r,
-                                           specialize=adjoint and specialize)
+                                           specialize=specialize)
         return self._programs[key]
 
     def configure(self, k, r, k_bwd=None, r_bwd=None):
@@ -336,7 +356,7 @@ class SVExecutor:
         # the adjoint holds psi and lambda registers: more than 16 amplitudes each would spill
         r_bwd = min(4, r if r_bwd is None else r_bwd)
         self.fwd = self._program(k, r)
-        self.bwd = self._program(k_bwd, r_bwd, adjoint=True)
+        self.bwd = self._program(k_bwd, r_bwd)
         if self.jit:
             self._ensure_jit(torch.device("cuda", torch.cuda.current_device()))
 

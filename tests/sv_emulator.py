@@ -15,7 +15,7 @@ import torch
 from oracle.sv_oracle import gate_matrix
 from tedq_amd.engine import program as P
 from tedq_amd.engine.planner import (OP_U1, OP_D1, OP_CU1, OP_CX, OP_CZ, OP_U2, OP_GRAD1, OP_CGRAD1, OP_CD1,
-                                     OP_R1, OP_PGRAD, OP_GFLUSH)
+                                     OP_R1, OP_PGRAD, OP_GFLUSH, OP_PH1)
 
 CODE_NAMES = {0: "I", 1: "Hadamard", 2: "PauliX", 3: "PauliY", 4: "PauliZ", 5: "S", 6: "T", 7: "SX",
               14: "RX", 15: "RY", 16: "RZ", 17: "Rot", 18: "PhaseShift"}
@@ -67,6 +67,8 @@ def build_tables(enc, plan, row):
         for U in Us:
             S = U @ S
         mats[moff:moff + 4] = S.reshape(-1)
+        if int(rec[6]):
+            mats[moff + 1] = S[1, 1] * np.conj(S[0, 0])
         scalar = int(rec[5])
         if scalar > 0:
             # single-value gradient in component c of the gslot: A picks it out of 2 Re sum A C
@@ -105,6 +107,13 @@ def _apply_ops(regs_list, ops, mats, acc):
         typ, dag = int(op[0]) & 0xFF, (int(op[0]) >> 8) & 1
         b0, b1, cmask, moff, gslot = int(op[1]), int(op[2]), int(op[3]), int(op[4]), int(op[5])
         if typ == OP_GFLUSH:
+            continue
+        if typ == OP_PH1:
+            ph = np.conj(mats[moff + 1]) if dag else mats[moff + 1]
+            for v in regs_list:
+                for j in range(R):
+                    if (j >> b0) & 1 and (j & cmask) == cmask:
+                        v[:, j] = ph * v[:, j]
             continue
         if typ == OP_PGRAD:
             p, l = regs_list

@@ -18,7 +18,9 @@ from tedq_amd.serving.engine import Request, ServingEngine, latency_stats
 pytestmark = pytest.mark.gpu
 
 
-def hea_z0(n, layers):
+def hea_z0(n, layers, fold=True):
+    """Chain ansatz measuring <Z_0>; fold=False keeps every gate and qubit (by default only the backward
+    light cone of Z_0, about ``layers`` qubits, is simulated)."""
     def cd(*p):
         it = iter(p)
         for _ in range(layers):
@@ -28,7 +30,7 @@ def hea_z0(n, layers):
             for q in range(n - 1):
                 qai.CNOT(qubits=[q, q + 1])
         return qai.expval(qai.PauliZ(qubits=[0]))
-    return qai.Circuit(cd, n, *([0.1] * (2 * n * layers))).compilecircuit(backend="pytorch")
+    return qai.Circuit(cd, n, *([0.1] * (2 * n * layers))).compilecircuit(backend="pytorch", fold=fold)
 
 
 def test_results_alone_equal_results_in_the_mix():
@@ -114,15 +116,15 @@ def test_graph_replay_matches_eager_across_replays():
 
 
 def test_large_requests_alone_equal_results_in_the_mix():
-    """Requests at the top of the serving stream's range (20-24 qubits, up to 20 layers) mixed with
-    small ones: each value and gradient equals the request evaluated alone (2e-6), and a 16-qubit
-    request equals the oracle (1e-5 relative)."""
+    """Requests at the top of the serving stream's range (20-24 qubits, up to 20 layers, every gate kept)
+    mixed with small ones: each value and gradient equals the request evaluated alone (2e-6), and a
+    16-qubit request equals the oracle (1e-5 relative)."""
     rng = np.random.default_rng(12)
     cache = {}
 
     def factory(key):
         if key not in cache:
-            cache[key] = hea_z0(*key)
+            cache[key] = hea_z0(*key, fold=False)
         return cache[key]
 
     keys = [(24, 20), (6, 3), (22, 12), (20, 20), (8, 2), (20, 20), (16, 5), (24, 2), (6, 3), (21, 7)]
@@ -161,7 +163,7 @@ def test_ragged_launches_equal_circuits_alone():
     items, params = [], []
     for key in keys:
         if key not in cache:
-            cache[key] = hea_z0(*key).executor()
+            cache[key] = hea_z0(*key, fold=False).executor()
         ex = cache[key]
         n, L = key
         B = int(rng.integers(1, 4))
@@ -194,7 +196,7 @@ def test_ragged_serving_mixes_structures_and_matches_alone():
 
     def factory(key):
         if key not in cache:
-            cache[key] = hea_z0(*key)
+            cache[key] = hea_z0(*key, fold=False)
         return cache[key]
 
     reqs = []

@@ -294,8 +294,9 @@ def test_generated_pass_kernels_match_interpreter_and_oracle(seed, n, tile, reg)
     spec = _random_spec(n, 50, rng)
     obs = [int(q) for q in rng.choice(n, size=3, replace=False)]
     meas_fn = lambda: [qai.expval(qai.PauliZ(qubits=[q])) for q in obs]  # noqa: E731
-    c_int, npar = _build(n, spec, meas_fn, tile_bits=tile, reg_bits=reg)
-    c_jit, _ = _build(n, spec, meas_fn, tile_bits=tile, reg_bits=reg, jit=True)
+    # fold=False: the whole n-qubit circuit runs, so generated kernels apply (k < n)
+    c_int, npar = _build(n, spec, meas_fn, tile_bits=tile, reg_bits=reg, fold=False)
+    c_jit, _ = _build(n, spec, meas_fn, tile_bits=tile, reg_bits=reg, jit=True, fold=False)
     vals = rng.uniform(-math.pi, math.pi, size=(5, npar))
     rows = torch.tensor(vals, dtype=torch.float32, device=DEV)
     ups = [torch.ones(5, device=DEV) for _ in obs]
@@ -617,16 +618,19 @@ def test_product_observable_with_many_non_z_factors():
 
 
 def test_specialised_adjoint_plan_on_the_interpreted_kernels():
-    """The adjoint plan with unfused rotations (real 2x2 ops, single-value gradients in groups of 8) runs
-    on the library's interpreted pass kernel as well as on generated ones: gradients against the oracle
-    (1e-5 relative) on a random circuit with every rotation kind, controlled and inverted."""
+    """The specialised plan (unfused rotations as real 2x2 ops, single-value gradients in groups of 8,
+    diagonal gates without their global phase) runs on the library's interpreted pass kernels as well as
+    on generated ones, forward and adjoint: values and gradients against the oracle (1e-5 relative) on a
+    random circuit with every rotation kind, controlled and inverted."""
     n = 11
     rng = np.random.default_rng(61)
     spec = _random_spec(n, 60, rng)
     obs = [0, 5, 9]
     c, npar = _build(n, spec, lambda: [qai.expval(qai.PauliZ(qubits=[q])) for q in obs], tile_bits=8, reg_bits=3)
     ex = c.executor()
-    ex.bwd = ex._program(ex.bwd.plan.k, ex.bwd.plan.r, adjoint=True, specialize=True)  # pylint: disable=protected-access
+    # pylint: disable=protected-access
+    ex.bwd = ex._program(ex.bwd.plan.k, ex.bwd.plan.r, specialize=True)
+    ex.fwd = ex._program(ex.fwd.plan.k, ex.fwd.plan.r, specialize=True)
     assert any(sl.scomp >= 0 for sl in ex.bwd.plan.slots)
     vals = rng.uniform(-math.pi, math.pi, size=(2, npar))
     rows = torch.tensor(vals, dtype=torch.float32, device=DEV)
@@ -634,6 +638,61 @@ def test_specialised_adjoint_plan_on_the_interpreted_kernels():
     t64 = torch.tensor(vals[1], dtype=torch.float64, requires_grad=True)
     ref = O.execute(n, _oracle_gates(spec, list(t64)), [("expval", ("PauliZ", [q])) for q in obs])
     ref.sum().backward()
+    e_v = relerr(np.array([float(o[1]) for o in outs]), ref.detach().numpy())
     e_g = relerr(grad[1].cpu().numpy(), t64.grad.numpy(), GFLOOR)
-    print(f"specialised adjoint on interpreted kernels: gradients relerr {e_g:.2e}", flush=True)
-    assert e_g < TOL
+    print(f"specialised plan on interpreted kernels: values relerr {e_v:.2e}, gradients relerr {e_g:.2e}",
+          flush=True)
+    assert e_v < TOL and e_g < TOL
+
+
+def test_folded_observables_equal_the_whole_circuit():
+    """Trailing gates folded into Z-string observables and unused qubits dropped (the default) give the
+    whole circuit's values and gradients: a 16-qubit, 5-layer chain ansatz measuring <Z_0> runs on the 5
+    qubits of its backward light cone; a random circuit with several Z-string products; both against fold=False and the oracle
+    (1e-5 relative)."""
+    import bench
+    n, L = 16, 5
+    # seed 78: all three Z-string values of the random circuit are away from 0 (relative errors)
+    rng = np.random.default_rng(78)
+    vals = rng.uniform(-math.pi, math.pi, size=(3, 2 * n * L))
+    rows = torch.tensor(vals, dtype=torch.float32, device=DEV)
+    folded = bench.hea_z0_circuit(n, L).compilecircuit(backend="pytorch").executor()
+    whole = bench.hea_z0_circuit(n, L).compilecircuit(backend="pytorch", fold=False).executor()
+    assert folded.n == L and whole.n == n and folded.n_circuit == n
+    v_f, g_f = folded.value_and_grad(rows, [torch.ones(3, device=DEV)])
+    v_w, g_w = whole.value_and_grad(rows, [torch.ones(3, device=DEV)])
+    t64 = torch.tensor(vals[2], dtype=torch.float64, requires_grad=True)
+    th = t64.reshape(L, n, 2)
+    gates = []
+    for layer in range(L):
+        for q in range(n):
+            gates += [("RY", [q], [th[layer, q, 0]]), ("RZ", [q], [th[layer, q, 1]])]
+        gates += [("CNOT", [q, q + 1], []) for q in range(n - 1)]
+    ref = O.execute(n, gates, [("expval", ("PauliZ", [0]))])
+    ref.sum().backward()
+    assert abs(float(v_f[0][2]) - float(ref[0])) < 1e-5 and abs(float(v_w[0][2]) - float(ref[0])) < 1e-5
+    e_f = relerr(g_f[2].cpu().numpy(), t64.grad.numpy(), GFLOOR)
+    e_w = relerr(g_w[2].cpu().numpy(), t64.grad.numpy(), GFLOOR)
+    print(f"folded <Z_0> on {folded.n} of {n} qubits: gradient relerr {e_f:.2e} (whole circuit {e_w:.2e})",
+          flush=True)
+    assert e_f < TOL and e_w < TOL
+    # gates outside the light cone have exactly zero gradient
+    assert int((g_f[2] == 0).sum()) >= n * L
+    spec = _random_spec(12, 60, rng) + [("CNOT", [q, q + 1], 0) for q in range(11)] + [("CZ", [0, 5], 0)]
+    obs = [[0, 3], [7], [2, 9, 11]]
+    meas = lambda: [qai.expval([qai.PauliZ(qubits=[q]) for q in o]) for o in obs]  # noqa: E731
+    c_f, npar = _build(12, spec, meas, tile_bits=8, reg_bits=3)
+    c_w, _ = _build(12, spec, meas, tile_bits=8, reg_bits=3, fold=False)
+    assert c_f.executor().folded >= 12
+    vals = rng.uniform(-math.pi, math.pi, size=(2, npar))
+    rows = torch.tensor(vals, dtype=torch.float32, device=DEV)
+    ups = [torch.ones(2, device=DEV) for _ in obs]
+    v_f, g_f = c_f.executor().value_and_grad(rows, ups)
+    v_w, g_w = c_w.executor().value_and_grad(rows, ups)
+    t64 = torch.tensor(vals[1], dtype=torch.float64, requires_grad=True)
+    ref = O.execute(12, _oracle_gates(spec, list(t64)), [("expval", [("PauliZ", [q]) for q in o]) for o in obs])
+    ref.sum().backward()
+    got = np.array([float(v[1]) for v in v_f])
+    assert relerr(got, ref.detach().numpy()) < TOL
+    assert relerr(g_f[1].cpu().numpy(), t64.grad.numpy(), GFLOOR) < TOL
+    assert np.max(np.abs(torch.stack(v_f).cpu().numpy() - torch.stack(v_w).cpu().numpy())) < 1e-5

@@ -107,14 +107,33 @@ def test_emulated_adjoint_gradient_matches_autograd(seed, n, k, r, specialize):
         z += d * (1 - 2 * ((np.arange(2 ** n) >> bit) & 1))
     lam = z * psi
     grad, psi0 = EMU.adjoint_grad(enc, plan, psi, lam, mats, amats, P)
-    # the reverse sweep must uncompute psi back to |0..0>
+    # the reverse sweep must uncompute psi back to |0..0> (the specialised plan's phase-free diagonal
+    # ops are undone exactly by their adjoint forms)
     e0 = np.zeros(2 ** n)
     e0[0] = 1
     assert np.allclose(psi0, e0, atol=1e-10)
+    if specialize:
+        # its forward state is the circuit's up to a global phase
+        ref = O.propagate(n, oracle_gates(spec, values)).numpy().reshape(-1)
+        ph = np.vdot(psi, ref)
+        assert abs(abs(ph) - 1) < 1e-10 and np.allclose(psi * ph, ref, atol=1e-10)
     t = torch.tensor(values, dtype=torch.float64, requires_grad=True)
     out = O.execute(n, oracle_gates(spec, list(t)), [("expval", ("PauliZ", [q])) for q in obs_q])
     (out * torch.tensor(dy)).sum().backward()
     assert np.allclose(grad, t.grad.numpy(), atol=1e-9)
+
+
+def _adjoint_op_codes(enc):
+    """Op codes (low byte of word 0) of every op in the encoded adjoint passes."""
+    from tedq_amd.engine import program as PR
+    codes = []
+    for off in enc.adj_offsets:
+        w = enc.prog[off:]
+        for s in range(int(w[2])):
+            st = w[PR.HDR_STAGES + PR.STAGE_WORDS * s:]
+            nops, ostart = int(st[0]), int(st[1])
+            codes += [int(w[ostart + o * PR.OP_WORDS]) & 0xFF for o in range(nops)]
+    return codes
 
 
 def test_trainable_subset_and_order():
@@ -186,8 +205,8 @@ def test_two_qubit_unitary_lowering():
 
 def test_specialized_adjoint_with_inverted_and_controlled_rotations():
     """The adjoint plan's single-value gradients (RX, RY, RZ, PhaseShift, their controlled blocks and
-    their inverses, which negate the value) and real 2x2 ops give the oracle's autograd gradient; more
-    than 8 such gradients in one register stage span several groups."""
+    their inverses, which negate the value), real 2x2 ops and phase-only diagonals give the oracle's
+    autograd gradient; more than 8 such gradients in one register stage span several groups."""
     n = 4
     rng = np.random.default_rng(7)
     names = ["RX", "RY", "RZ", "PhaseShift", "CRX", "CRY", "CRZ", "ControlledPhaseShift", "Hadamard", "CNOT"]
@@ -214,6 +233,11 @@ def test_specialized_adjoint_with_inverted_and_controlled_rotations():
     assert sum(1 for sl in plan.slots if sl.scomp >= 0) == npar
     assert max(sum(1 for o in st.ops if o[4] >= 0 and plan.slots[o[4]].scomp >= 0)
                for p in plan.passes for st in p.stages) > 8
+    # uncontrolled RZ / PhaseShift run as phases on the |1> half (no global phase), controlled ones as
+    # controlled diagonals
+    codes = _adjoint_op_codes(enc)
+    n_diag = sum(1 for name, _, _ in spec if name in ("RZ", "PhaseShift"))
+    assert n_diag > 0 and codes.count(13) == n_diag and 2 not in codes
     psi, mats, amats = EMU.forward(enc, plan, np.asarray(values))
     lam = (1 - 2 * ((np.arange(2 ** n) >> (n - 2)) & 1)) * psi
     grad, _ = EMU.adjoint_grad(enc, plan, psi, lam, mats, amats, P)
@@ -253,3 +277,63 @@ def test_basis_expansion_of_many_factor_observables():
     V = embed({b: v for b, v in spec.basis})
     zsum = sum(c * embed({b: Z for b in range(n) if (m >> b) & 1}) for m, c in zip(spec.bmasks, spec.bcoefs))
     assert np.allclose(V @ zsum @ V.conj().T, dense, atol=1e-12)
+
+
+def _folded_case(n, spec, values, observables):
+    """Executor of the traced circuit: trailing gates folded into the Z-string observables, unused
+    qubits dropped."""
+    def meas():
+        return [qai.expval([qai.PauliZ(qubits=[q]) for q in obs]) for obs in observables]
+    cc = trace(n, spec, values, meas).compilecircuit(backend="pytorch")
+    ex = cc.executor()
+    return ex
+
+
+@pytest.mark.parametrize("seed,n,observables", [(20, 6, [[0], [2, 4]]), (21, 7, [[3]]), (22, 5, [[1, 2, 3], [4]]),
+                                                (23, 8, [[0]])])
+def test_folded_trailing_gates_keep_values_and_gradients(seed, n, observables):
+    """Trailing CNOTs, CZs, diagonal and out-of-light-cone gates folded into Z-string observables, and
+    qubits left without gates dropped: the emulated values and adjoint gradients of the folded program
+    equal the oracle's on the whole circuit (gradients of dropped gates are exactly zero there too)."""
+    rng = np.random.default_rng(seed)
+    spec = random_spec(n, 20, rng)
+    # a hardware-efficient tail: rotation layer and CNOT chain, then CZ and diagonal gates
+    for q in range(n):
+        spec += [("RY", [q], 1), ("RZ", [q], 1)]
+    spec += [("CNOT", [q, q + 1], 0) for q in range(n - 1)]
+    spec += [("CZ", [0, n - 1], 0), ("PhaseShift", [1], 1), ("T", [2], 0), ("CRZ", [3, 0], 1)]
+    npar = sum(s[2] for s in spec)
+    values = [float(x) for x in rng.uniform(-math.pi, math.pi, npar)]
+    ex = _folded_case(n, spec, values, observables)
+    assert ex.folded >= n + 4
+    plan = build_plan(ex.n, ex._units, npar, list(ex._pool), k_max=min(ex.n, 4), r=2,
+                      specialize=True)
+    enc = encode_plan(plan)
+    psi, mats, amats = EMU.forward(enc, plan, np.asarray(values))
+    idx = np.arange(2 ** ex.n)
+    zs = [1 - 2 * (np.array([bin(int(x) & m).count("1") for x in idx]) & 1) for m in ex.zmasks]
+    vals = [float(np.sum(z * np.abs(psi) ** 2)) for z in zs]
+    t = torch.tensor(values, dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, oracle_gates(spec, list(t)),
+                    [("expval", [("PauliZ", [q]) for q in obs]) for obs in observables])
+    assert np.allclose(vals, ref.detach().numpy(), atol=1e-10)
+    dy = rng.normal(size=len(observables))
+    lam = sum(d * z for d, z in zip(dy, zs)) * psi
+    grad, _ = EMU.adjoint_grad(enc, plan, psi, lam, mats, amats, npar)
+    (ref * torch.tensor(dy)).sum().backward()
+    assert np.allclose(grad, t.grad.numpy(), atol=1e-9)
+
+
+def test_fold_keeps_everything_for_non_z_measurements():
+    """A probability or non-Z measurement disables folding; a single <Z_0> on a 10-qubit chain ansatz
+    keeps only its backward light cone and the qubits in it."""
+    import bench
+    spec = [("RY", [q], 1) for q in range(4)] + [("CNOT", [q, q + 1], 0) for q in range(3)]
+    cc = trace(4, spec, [0.1] * 4, lambda: qai.probs(qubits=[0])).compilecircuit(backend="pytorch")
+    assert cc.executor().folded == 0 and cc.executor().n == 4
+    cc = trace(4, spec, [0.1] * 4, lambda: qai.expval(qai.PauliX(qubits=[0]))).compilecircuit(backend="pytorch")
+    assert cc.executor().folded == 0
+    ex = bench.hea_z0_circuit(10, 3).compilecircuit(backend="pytorch").executor()
+    # last layer: RY on qubit 0; layer 2: rotations on qubits 0-1 and CNOT(0,1); layer 1: rotations on
+    # qubits 0-2 and CNOT(0,1), CNOT(1,2)
+    assert ex.n == 3 and len(ex._units) == 1 + (4 + 1) + (6 + 2)
