@@ -238,8 +238,10 @@ __global__ void k_build_tables(const PT* __restrict__ params, int64_t batch, int
 // builds the same operands with torch ops per gate per call
 // (tedq/backends/pytorch_backend.py:524-546 for the conjugated copies).
 // ---------------------------------------------------------------------------
-// diag = 1: the tensor is stored as its D diagonal entries (a diagonal gate whose wires the network
-// merged into hyper-indices) instead of the D x D matrix.
+// diag = w > 0: the network merged the gate's first w wires (in qubit order) into hyper-indices, the
+// wires along which its matrix is diagonal.  w = nq: the tensor is its D diagonal entries; 0 < w < nq
+// (the controls of a controlled gate): entry [c][t'][t] = M[(c, t'), (c, t)] over the w merged bits c
+// and the remaining out / in bits t', t, 2^w x 4^(nq - w) entries; w = 0: the D x D matrix.
 struct TNGateRec {
   int code, src0, src1, src2, const_off, nq, kind, bra, off, d0, d1, d2, dagger, diag, pad2, pad3;
 };
@@ -257,10 +259,20 @@ __global__ void k_tn_gate_tensors(const PT* __restrict__ params, int64_t batch, 
   float2* out = leaves + b * leaf_stride + r.off;
   const int D = 1 << r.nq;
   if (r.kind == 1) {  // constant matrix, row-major complex128
-    if (r.diag) {
+    if (r.diag == r.nq) {
       for (int i = 0; i < D; ++i) {
         const double2 v = cpool[r.const_off + i * D + i];
         out[i] = make_float2((float)v.x, (float)(r.bra ? -v.y : v.y));
+      }
+      return;
+    }
+    if (r.diag > 0) {
+      const int f = r.nq - r.diag, F = 1 << f;       // free (unmerged) wires
+      for (int e = 0; e < (1 << r.diag) * F * F; ++e) {
+        const int c = e / (F * F), tp = (e / F) % F, t = e % F;
+        const int i = c * F + tp, j = c * F + t;
+        const double2 v = r.bra ? cpool[r.const_off + j * D + i] : cpool[r.const_off + i * D + j];
+        out[e] = make_float2((float)v.x, (float)(r.bra ? -v.y : v.y));
       }
       return;
     }
@@ -282,11 +294,20 @@ __global__ void k_tn_gate_tensors(const PT* __restrict__ params, int64_t batch, 
   const M2 V = comp_eval(cr, prow, cpool, -1, &ok);
   if (!ok) atomicOr(err, 2);
   // controlled embedding: identity except the block where every control is 1
-  if (r.diag) {
+  if (r.diag == r.nq) {
     for (int i = 0; i < D; ++i) {
       const int bi = i - (D - 2);
       const cd v = bi >= 0 ? (r.bra ? cd_conj(V.a[bi * 3]) : V.a[bi * 3]) : mkcd(1.0, 0.0);
       out[i] = make_float2((float)v.x, (float)v.y);
+    }
+  } else if (r.diag > 0) {
+    // controls merged (r.diag = nq - 1): [c][t'][t], the target block where every control is 1
+    const int last = (1 << r.diag) - 1;
+    for (int e = 0; e < (4 << r.diag); ++e) {
+      const int c = e >> 2, a = (e >> 1) & 1, b2 = e & 1;
+      cd v = mkcd(a == b2 ? 1.0 : 0.0, 0.0);
+      if (c == last) v = r.bra ? cd_conj(V.a[b2 * 2 + a]) : V.a[a * 2 + b2];
+      out[e] = make_float2((float)v.x, (float)v.y);
     }
   }
   for (int i = 0; i < D && !r.diag; ++i)
@@ -328,6 +349,7 @@ __global__ void k_tn_param_grad(const float2* __restrict__ env, int64_t env_stri
   if (r.dslot < 0) return;
   const float2* dv = dmats + b * dmat_stride + (int64_t)r.dslot * 4;
   const bool diag = r.nq & 16;   // leaves stored as diagonals: entry (i, i) at position i
+  const bool ctrl = r.nq & 32;   // controls merged: target block entry (a, c) at (2^(nq-1) - 1) * 4 + 2a + c
   const int D = 1 << (r.nq & 15);
   double acc = 0.0;
   if (diag) {
@@ -348,7 +370,7 @@ __global__ void k_tn_param_grad(const float2* __restrict__ env, int64_t env_stri
   }
   for (int a = 0; a < 2; ++a)
     for (int c = 0; c < 2; ++c) {
-      const int e = (D - 2 + a) * D + (D - 2 + c);
+      const int e = ctrl ? (D / 2 - 1) * 4 + a * 2 + c : (D - 2 + a) * D + (D - 2 + c);
       const float2 d = dv[a * 2 + c];
       if (r.ket_off >= 0) {
         const float2 x = env[b * env_stride + r.ket_off + e];

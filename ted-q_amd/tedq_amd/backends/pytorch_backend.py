@@ -67,10 +67,12 @@ class _AdjointFunction(torch.autograd.Function):
         if psi_sink is not None:
             psi_sink.append(psi)
         ctx.executor = executor
+        ctx.save_for_backward(rows)
         if isinstance(psi, torch.Tensor):
-            ctx.save_for_backward(rows, psi)
+            # the adjoint sweep walks psi back in place: the first backward consumes it (no copy), a
+            # second backward through the same graph recomputes the forward
+            ctx.psi = psi
         else:                               # contraction mode: a run record, recomputed in backward
-            ctx.save_for_backward(rows)
             ctx.run = psi
         res = []
         for spec, o in zip(executor.specs, outs):
@@ -89,9 +91,11 @@ class _AdjointFunction(torch.autograd.Function):
             (rows,) = ctx.saved_tensors
             grad = ex.run_backward(rows, ctx.run, ex.seed_adjoint(ctx.run, grads), grad_dtype=rows.dtype)
             return None, grad, None
-        rows, psi = ctx.saved_tensors
+        (rows,) = ctx.saved_tensors
+        psi = ctx.psi if ctx.psi is not None else ex.run_forward(rows)
+        ctx.psi = None
         lam = ex.seed_adjoint(psi, grads)
-        grad = ex.run_backward(rows, psi.clone(), lam, grad_dtype=rows.dtype)
+        grad = ex.run_backward(rows, psi, lam, grad_dtype=rows.dtype)
         return None, grad, None
 
 

@@ -47,20 +47,40 @@ _BLOCK_CODE = {"RX": 14, "RY": 15, "RZ": 16, "Rot": 17, "PhaseShift": 18,
 _DIAGONAL_GATES = {"RZ", "PhaseShift", "PauliZ", "S", "T", "I", "CZ", "CRZ", "ControlledPhaseShift"}
 
 
+# controlled gates: their leading (control) wires are diagonal for every parameter value
+_CONTROL_WIRES = {"CNOT": 1, "CY": 1, "CRX": 1, "CRY": 1, "Toffoli": 2, "CSWAP": 1}
+
+
 def _is_diagonal_matrix(m):
     m = np.asarray(m)
     return m.ndim == 2 and m.shape[0] == m.shape[1] and not np.any(m - np.diag(np.diag(m)))
 
 
-def diagonal_gate(op):
-    """True when the operator's matrix is diagonal whatever its parameters (by type), or it is a
-    constant diagonal matrix."""
+def _leading_diagonal_wires(m, nq):
+    """Leading wires (qubit order, wire 0 = most significant bit) in which the matrix is diagonal: its
+    entries vanish wherever row and column differ in that wire's bit."""
+    m = np.asarray(m).reshape(1 << nq, 1 << nq)
+    idx = np.arange(1 << nq)
+    for w in range(nq):
+        bit = 1 << (nq - 1 - w)
+        differ = ((idx[:, None] ^ idx[None, :]) & bit) != 0
+        if np.any(m[differ]):
+            return w
+    return nq
+
+
+def diagonal_wires(op):
+    """Number of leading wires in which the operator's matrix is diagonal whatever its parameters: all of
+    them for diagonal gates, the controls of controlled gates (by type), else from a constant matrix."""
     name = op.name[:-4] if getattr(op, "is_inverse", False) else op.name
+    nq = len(op.qubits)
     if name in _DIAGONAL_GATES:
-        return True
+        return nq
+    if name in _CONTROL_WIRES:
+        return _CONTROL_WIRES[name]
     if op.num_params == 0:
-        return _is_diagonal_matrix(np.asarray(op.matrix).reshape(2 ** len(op.qubits), -1))
-    return False
+        return _leading_diagonal_wires(op.matrix, nq)
+    return 0
 _NO_SRC = -2 ** 31
 _DESC_WORDS = 4 + 9 * 32
 _ROLE = {"AH": 0, "AM": 1, "AK": 2, "BH": 3, "BK": 4, "BN": 5, "CH": 6, "CM": 7, "CN": 8}
@@ -185,16 +205,16 @@ class _Network:
                  searched=None):
         # an externally found order applies to the network as built (no light-cone reduction)
         red = tn if external is not None else cancel_inverse_pairs(tn)
-        diag = [False] * red.num_tensors
+        diag = [0] * red.num_tensors
         if is_diagonal is not None and external is None:
             red, diag = diagonal_reduce(red, is_diagonal)
-        self.n_diagonal = sum(diag)
+        self.n_diagonal = sum(1 for d in diag if d)
         inputs = [list(t) for t in red.input_indices]
         sources = list(red.sources)
         if len(inputs) < 2:                       # keep at least one step
             inputs.append([])
             sources.append(("one",))
-            diag.append(False)
+            diag.append(0)
             if external is not None:
                 external = [(0, 1)]
         self.measurement = tn.measurement
@@ -218,7 +238,7 @@ class _Network:
         self.sel_off = -1          # first constant slot of the selector vectors (set by the executor)
         if sliced:
             inputs, sources, path = slice_network(inputs, sources, self.output, path, sliced)
-            diag += [False] * (len(inputs) - len(diag))
+            diag += [0] * (len(inputs) - len(diag))
             sets = path_sets(inputs, self.output, path)
             largest, _ = path_stats(path, sets, len(inputs))
             info = dict(info, sliced=list(sliced), slices=1 << len(sliced), largest_rank=largest,
@@ -337,11 +357,12 @@ class TNExecutor:
         self.fused_backward = True
         self._small = {}
         ext = _external_paths(cc, paths)
-        # diagonal reduction (the reference's tn_simplify, by gate type): diagonal gates are stored as their
-        # diagonals and their wires merged into hyper-indices.  Off for externally given orders, which
-        # refer to the network as built; otherwise used when the modelled device time of the searched
-        # paths (summed over the networks) is lower with it than without it
-        self._diag_ops = {idx: diagonal_gate(op) for idx, op in cc._operators.items()}  # pylint: disable=protected-access
+        # diagonal reduction (the reference's tn_simplify, by gate type): the wires along which gates are
+        # diagonal (all of a diagonal gate's, the controls of a controlled gate) are merged into
+        # hyper-indices and the tensors stored without the repeated entries.  Off for externally given
+        # orders, which refer to the network as built; otherwise used when the modelled device time of the
+        # searched paths (summed over the networks) is lower with it than without it
+        self._diag_ops = {idx: diagonal_wires(op) for idx, op in cc._operators.items()}  # pylint: disable=protected-access
         self._searched = None
         self.simplify = False
         self.simplify_report = None
@@ -381,17 +402,21 @@ class TNExecutor:
                 pool.extend(mat.reshape(-1).tolist())
                 code, kind = 0, 1
             dag = int(bool(getattr(op, "is_inverse", False)))
-            diag = self.simplify and self._diag_ops[idx]
-            size = D if diag else D * D          # a diagonal leaf stores its D diagonal entries
+            # merged leading wires: a leaf keeps 2^dw x 4^(nq - dw) entries (D when fully diagonal)
+            dw = self._diag_ops[idx] if self.simplify else 0
+            if kind == 0 and 0 < dw < nq - 1:
+                raise RuntimeError(f"{op.name}: a parametrised gate is diagonal in all wires or its controls")
+            size = (1 << dw) << (2 * (nq - dw))
             ket, bra = off, off + size
             off += 2 * size
             self._gate_off[idx] = (ket, bra)
             for is_bra, o in ((0, ket), (1, bra)):
                 d = dsl if not is_bra else [-1, -1, -1]
                 recs.append([code, srcs[0], srcs[1], srcs[2], coff, nq, kind, is_bra, o, d[0], d[1], d[2],
-                             dag, int(diag), 0, 0])
+                             dag, dw, 0, 0])
+            flag = 16 if dw == nq else 32 if dw else 0      # leaf layout: diagonal / controls merged / full
             for i, _ in enumerate(trainable):
-                param_recs.append([ket, bra, nq | (16 if diag else 0), flat + i])
+                param_recs.append([ket, bra, nq | flag, flat + i])
             flat += len(trainable)
         self.n_params = flat
         self.leaf_stride = max(off, 1)
@@ -493,16 +518,18 @@ class TNExecutor:
         self._dev = {}
 
     def _is_diagonal(self, meas):
-        """Predicate on network sources: stored as a diagonal (gates by type, observables by matrix)."""
+        """Merged leading wires of network sources (gates by type, observables when diagonal)."""
         def pred(s):
             if s[0] in ("gate", "adj"):
                 return self._diag_ops[s[1]]
             if s[0] == "obs":
-                return _is_diagonal_matrix(meas.obs[s[1]].matrix)
-            if s[0] == "obsmat":
+                m = np.asarray(meas.obs[s[1]].matrix)
+            elif s[0] == "obsmat":
                 m = np.asarray(meas.obs.matrix)
-                return _is_diagonal_matrix(m.reshape(int(np.sqrt(m.size)), -1))
-            return False
+            else:
+                return 0
+            m = m.reshape(int(np.sqrt(m.size)), -1)
+            return int(np.log2(m.shape[0])) if _is_diagonal_matrix(m) else 0
         return pred
 
     def _choose_reduction(self, cc, trials):

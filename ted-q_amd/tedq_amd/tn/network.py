@@ -171,21 +171,25 @@ def cancel_inverse_pairs(tn):
     return TensorNetwork(fin_ix, list(tn.output_indices), size, fin_src, tn.measurement)
 
 
-def diagonal_reduce(tn, is_diagonal):
-    """Collapse the wires through diagonal tensors into hyper-indices.
+def diagonal_reduce(tn, diagonal_wires):
+    """Collapse the wires along which tensors are diagonal into hyper-indices.
 
-    A tensor that is diagonal as a matrix (RZ, PhaseShift, Z, S, T, CZ, CRZ,
-    controlled phase, diagonal observables and user matrices, and their bra
-    copies) maps each wire to itself, so its output index new_i and input
-    index old_i carry the same value: they are merged into one index and the
-    tensor keeps only its diagonal, [new..., old...] -> [w...].  Chains of
-    diagonal gates on a wire become one index shared by all of them.  The
-    reference's TensorNetwork.diagonal_reduce (tensor_network.py:316-392)
-    decides this from the trace-time array values, which changes meaning when
-    a parameter later takes another value; here ``is_diagonal(source)``
-    decides it from the gate type (diagonal for every parameter value) or from
-    a constant matrix.  Output indices keep their names.  Returns
-    (network, diag) with diag[t] True for the tensors stored as diagonals.
+    A gate tensor [new..., old...] that is diagonal in wire i (its matrix
+    entries vanish wherever the row and column differ in that wire's bit)
+    carries the same value on new_i and old_i: the two indices merge into one.
+    All wires of a diagonal gate (RZ, PhaseShift, Z, S, T, CZ, CRZ, controlled
+    phase, diagonal observables and user matrices) merge, and the tensor keeps
+    its diagonal, [w...]; the control wires of a controlled gate (CNOT, CY,
+    CRX, CRY, Toffoli, CSWAP) merge and the tensor becomes
+    [w_controls..., new_targets..., old_targets...].  Chains of such wires
+    become one index shared by all tensors on them.  The reference's
+    TensorNetwork.diagonal_reduce (tensor_network.py:316-392) finds diagonal
+    axis pairs from the trace-time array values, which changes meaning when a
+    parameter later takes another value; here ``diagonal_wires(source)`` gives
+    the number of leading wires (in the gate's qubit order) that are diagonal
+    for every parameter value, from the gate type or a constant matrix.
+    Output indices keep their names.  Returns (network, diag) with diag[t] the
+    number of merged leading wires of tensor t (0: unchanged).
     """
     parent = {}
 
@@ -197,12 +201,14 @@ def diagonal_reduce(tn, is_diagonal):
     out = set(tn.output_indices)
     diag = []
     for t, (ix, s) in enumerate(zip(tn.input_indices, tn.sources)):
-        d = s[0] in ("gate", "adj", "obs", "obsmat") and len(ix) % 2 == 0 and len(ix) > 0 and is_diagonal(s)
+        d = 0
+        if s[0] in ("gate", "adj", "obs", "obsmat") and len(ix) % 2 == 0 and len(ix) > 0:
+            d = min(int(diagonal_wires(s)), len(ix) // 2)
         diag.append(d)
         if not d:
             continue
         k = len(ix) // 2
-        for a, b in zip(ix[:k], ix[k:]):
+        for a, b in zip(ix[:d], ix[k:k + d]):
             ra, rb = root(a), root(b)
             if ra == rb:
                 continue
@@ -215,7 +221,7 @@ def diagonal_reduce(tn, is_diagonal):
     for ix, d in zip(tn.input_indices, diag):
         if d:
             k = len(ix) // 2
-            new_ix.append([root(x) for x in ix[k:]])
+            new_ix.append([root(x) for x in ix[k:k + d]] + [root(x) for x in ix[d:k]] + [root(x) for x in ix[k + d:]])
         else:
             new_ix.append([root(x) for x in ix])
     size = {x: 2 for t in new_ix for x in t}

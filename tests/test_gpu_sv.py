@@ -696,3 +696,18 @@ def test_folded_observables_equal_the_whole_circuit():
     assert relerr(got, ref.detach().numpy()) < TOL
     assert relerr(g_f[1].cpu().numpy(), t64.grad.numpy(), GFLOOR) < TOL
     assert np.max(np.abs(torch.stack(v_f).cpu().numpy() - torch.stack(v_w).cpu().numpy())) < 1e-5
+
+
+def test_second_backward_through_the_same_graph():
+    """The adjoint sweep consumes the forward state in place (no copy); a second backward through the same
+    graph (retain_graph) recomputes the forward and gives the same gradient."""
+    from tedq_amd import workloads as W
+    n, L = 8, 3
+    rng = np.random.default_rng(81)
+    c = W.hea(n, L)
+    rows = torch.tensor(rng.uniform(0, 2 * np.pi, size=(4, 2 * n * L)), dtype=torch.float32, device=DEV,
+                        requires_grad=True)
+    out = c.execute_batch(rows)
+    g1 = torch.autograd.grad(out.sum(), rows, retain_graph=True)[0]
+    g2 = torch.autograd.grad(out.sum(), rows)[0]
+    assert torch.equal(g1, g2)

@@ -502,19 +502,21 @@ def path_sets_of(net):
 @pytest.mark.parametrize("seed", range(4))
 def test_diagonal_reduction_keeps_values_and_gradients(seed):
     """tn_simplify: diagonal gates (RZ, PhaseShift, CRZ, controlled phase, CZ, Z, S, T, a constant diagonal
-    Unitary, diagonal observables) are stored as diagonals and their wires merged into hyper-indices;
+    Unitary, diagonal observables) are stored as diagonals and their wires merged into hyper-indices, and
+    the control wires of controlled gates (CNOT, CRX, CRY, Toffoli, CSWAP) are merged the same way;
     values and gradients equal the unsimplified network's and the oracle's, for expectation values and
     probabilities (output indices keep their names)."""
     rng = np.random.default_rng(100 + seed)
     n = 5
     diag_u = np.diag(np.exp(1j * rng.uniform(0, 6, 4)))
     names = ["RX", "RZ", "PhaseShift", "CRZ", "ControlledPhaseShift", "CZ", "PauliZ", "S", "T", "Hadamard",
-             "CNOT", "RY", "U2d"]
-    npar = {"RX": 1, "RZ": 1, "PhaseShift": 1, "CRZ": 1, "ControlledPhaseShift": 1, "RY": 1}
+             "CNOT", "RY", "U2d", "CRX", "CRY", "Toffoli", "CSWAP"]
+    npar = {"RX": 1, "RZ": 1, "PhaseShift": 1, "CRZ": 1, "ControlledPhaseShift": 1, "RY": 1, "CRX": 1, "CRY": 1}
     spec = []
     for _ in range(24):
         name = names[rng.integers(len(names))]
-        k = 2 if name in ("CRZ", "ControlledPhaseShift", "CZ", "CNOT", "U2d") else 1
+        k = (3 if name in ("Toffoli", "CSWAP") else
+             2 if name in ("CRZ", "ControlledPhaseShift", "CZ", "CNOT", "U2d", "CRX", "CRY") else 1)
         spec.append((name, [int(q) for q in rng.choice(n, size=k, replace=False)]))
     P = sum(npar.get(nm, 0) for nm, _ in spec)
     q0, q1 = [int(x) for x in rng.choice(n, size=2, replace=False)]
@@ -540,6 +542,13 @@ def test_diagonal_reduction_keeps_values_and_gradients(seed):
         ex = install(TNExecutor(cc, diagonal_reduction=simplify))
         if simplify:
             assert sum(net.n_diagonal for net in ex.networks) > 0
+            # every controlled gate's leaf lost its control pairs
+            for idx, op in cc._operators.items():  # pylint: disable=protected-access
+                name = op.name[:-4] if getattr(op, "is_inverse", False) else op.name
+                if name in ("CNOT", "CRX", "CRY", "CSWAP"):
+                    assert ex._diag_ops[idx] == 1  # pylint: disable=protected-access
+                elif name == "Toffoli":
+                    assert ex._diag_ops[idx] == 2  # pylint: disable=protected-access
         ups = [torch.ones(2) if s.kind == "fexp" else torch.ones(2, 4) for s in ex.specs]
         res[simplify] = ex.value_and_grad(rows, ups)
     (v1, g1), (v0, g0) = res[True], res[False]

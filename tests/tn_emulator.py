@@ -54,6 +54,16 @@ def _view(t, stride, off, size, batch):
     return rows[:, off:off + size]
 
 
+def _pack(T, nq, dw):
+    """Leaf entries of a gate matrix whose first dw wires are merged: the diagonal (dw = nq), the
+    [merged][out][in] array (0 < dw < nq) or the whole matrix."""
+    if dw == nq:
+        return np.diag(T)
+    f = nq - dw
+    blocks = T.reshape(1 << dw, 1 << f, 1 << dw, 1 << f)
+    return np.stack([blocks[c, :, c, :] for c in range(1 << dw)]).reshape(-1)
+
+
 def install(ex):
     """Patch ``ex`` (a TNExecutor) to run its device calls on the CPU."""
 
@@ -70,10 +80,8 @@ def install(ex):
                 if kind == 1:
                     U = pool[coff:coff + D * D].reshape(D, D)
                     T = U.conj().T if bra else U
-                    if diag:
-                        lv[b, off:off + D] = np.diag(T)
-                    else:
-                        lv[b, off:off + D * D] = T.reshape(-1)
+                    v = _pack(T, nq, diag)
+                    lv[b, off:off + v.size] = v
                     continue
 
                 def val(s):
@@ -88,10 +96,8 @@ def install(ex):
                 U = np.eye(D, dtype=np.complex128)
                 U[D - 2:, D - 2:] = V
                 T = U.conj().T if bra else U
-                if diag:
-                    lv[b, off:off + D] = np.diag(T)
-                else:
-                    lv[b, off:off + D * D] = T.reshape(-1)
+                v = _pack(T, nq, diag)
+                lv[b, off:off + v.size] = v
                 for w, d in enumerate((d0, d1, d2)):
                     if d < 0:
                         continue
@@ -144,12 +150,17 @@ def install(ex):
         g = grad.numpy()
         for p, (ket, bra, nq, ds) in enumerate(ex._param_recs):  # pylint: disable=protected-access
             diag = bool(int(nq) & 16)
+            ctrl = bool(int(nq) & 32)
             D = 1 << (int(nq) & 15)
             for b in range(B):
                 dV = dm[b, ds * 4:ds * 4 + 4].astype(np.complex128).reshape(2, 2)
                 if diag:
                     Ek = np.diag(env[b, ket + D - 2:ket + D].astype(np.complex128))
                     Eb = np.diag(env[b, bra + D - 2:bra + D].astype(np.complex128))
+                elif ctrl:
+                    # [controls][out][in]: the block where every control is 1 is the last 2 x 2
+                    Ek = env[b, ket:ket + 2 * D].reshape(-1, 2, 2)[-1].astype(np.complex128)
+                    Eb = env[b, bra:bra + 2 * D].reshape(-1, 2, 2)[-1].astype(np.complex128)
                 else:
                     Ek = env[b, ket:ket + D * D].reshape(D, D)[D - 2:, D - 2:]
                     Eb = env[b, bra:bra + D * D].reshape(D, D)[D - 2:, D - 2:]
