@@ -9,9 +9,10 @@ inputs are resident in HBM when the timed region starts.
 
 Extra fields on the JSON line:
 * roofline: the dominant pass kernel, algorithmic bytes per launch (16 B per
-  amplitude for a forward pass, 32 B for an adjoint pass over psi and lambda)
-  divided by its mean launch time, measured with events on the launch stream
-  during the timed steps, against 8 TB/s;
+  amplitude for a forward pass, 32 B for an adjoint pass over psi and lambda,
+  over the tiles the pass processes: from |0..0> the first passes skip the
+  tiles that hold zeros) divided by its mean launch time, measured with events
+  on the launch stream during the timed steps, against 8 TB/s;
 * cpu_baseline: the complex64 CPU restatement of the reference algorithm
   (tensordot + permute per gate, torch autograd backward) on the host cores,
   on a bounded sample of the same workload;
@@ -540,12 +541,18 @@ def main():
 
     amps = B * (2 ** n)
     kernels = {}
+    # algorithmic bytes count only the tiles a pass processes: from |0..0> the first passes skip the tiles
+    # that hold zeros (zero masks), so their launches move a fraction of the state
+    live = {"sv_pass_fwd": [2.0 ** -bin(z).count("1") for z in ex.fwd.zero_masks],
+            "sv_pass_adj": [2.0 ** -bin(z).count("1") for z in ex.bwd.zero_masks]}
     for kind, bpa in (("sv_pass_fwd", 16), ("sv_pass_adj", 32)):
         ms, cnt = probe.mean_ms(kind)
         if ms:
-            gbs = bpa * amps / (ms * 1e-3) / 1e9
+            frac = float(np.mean(live[kind]))
+            gbs = bpa * amps * frac / (ms * 1e-3) / 1e9
             kernels[kind] = {"mean_ms": ms, "launches": cnt, "total_ms": ms * cnt,
-                             "bytes_per_launch": bpa * amps, "achieved_GBs": gbs}
+                             "bytes_per_launch": bpa * amps * frac, "live_tile_fraction": frac,
+                             "achieved_GBs": gbs}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     kd = kernels[dom]
     line = {

@@ -54,19 +54,26 @@ int tedq_sv_build_tables(const void* params, int params_are_f64, int64_t batch, 
 /* One forward pass (a tile of 2^tile_bits amplitudes per workgroup, register
  * stages of 2^reg_bits amplitudes per thread) over every circuit; dev_prog is
  * the pass program (int32 words, csrc/sv_passes.hip), tile_mask its physical
- * tile bits.  A sequence of passes replaces the SV propagation loop
- * tensordot + permute per gate (pytorch_backend.py:358-380). */
+ * tile bits.  zero_mask: non-tile bits on which the state is known to vanish
+ * (a circuit started in |0..0> whose earlier passes never touched them);
+ * tiles with any of these bits set are skipped and keep their (zero)
+ * contents; 0 processes every tile.  A sequence of passes replaces the SV
+ * propagation loop tensordot + permute per gate (pytorch_backend.py:358-380). */
 int tedq_sv_apply_pass(void* state, int64_t batch, int n_qubits, int tile_bits, int reg_bits, int threads_log2,
-                       const int* dev_prog, uint64_t tile_mask, const void* mats, int64_t mat_stride,
-                       void* stream);
+                       const int* dev_prog, uint64_t tile_mask, uint64_t zero_mask, const void* mats,
+                       int64_t mat_stride, void* stream);
 /* One adjoint pass: applies the daggered gates of a pass in reverse order to
  * psi and lam and writes the gradient bilinears of each parametrised slot to
- * partials [batch][n_gslots][partial_rows][8] (float).  Replaces torch
- * autograd through the SV loop (the default back_prop method,
- * pytorch_backend.py:142-144). */
+ * partials [batch][n_gslots][partial_rows][8] (float).  zero_mask as for
+ * tedq_sv_apply_pass (psi vanishes there during this pass, and no later pass
+ * reads lam there): skipped tiles write no partial rows (rows from
+ * 2^popcount(rest bits outside zero_mask) x rows-per-tile on stay as the
+ * caller set them).  Replaces torch autograd through the SV loop (the
+ * default back_prop method, pytorch_backend.py:142-144). */
 int tedq_sv_adjoint_pass(void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits, int reg_bits,
-                         int threads_log2, const int* dev_prog, uint64_t tile_mask, const void* mats,
-                         int64_t mat_stride, float* partials, int64_t partial_rows, int n_gslots, void* stream);
+                         int threads_log2, const int* dev_prog, uint64_t tile_mask, uint64_t zero_mask,
+                         const void* mats, int64_t mat_stride, float* partials, int64_t partial_rows, int n_gslots,
+                         void* stream);
 /* One pass for circuits of DIFFERENT structures and sizes in one launch (a
  * serving batch), forward (adjoint = 0) or adjoint.  dev_recs: device array of
  * n_recs 64-byte records {psi, lam, prog, mats, partials (device pointers of
@@ -163,8 +170,8 @@ int tedq_jit_unload(void* module);
  * arguments as tedq_sv_apply_pass / tedq_sv_adjoint_pass (k < n <= 28, at most
  * 64 KiB of LDS per workgroup). */
 int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits,
-                  int reg_bits, uint64_t tile_mask, const void* mats, int64_t mat_stride, void* partials,
-                  int64_t partial_rows, int n_gslots, void* stream);
+                  int reg_bits, uint64_t tile_mask, uint64_t zero_mask, const void* mats, int64_t mat_stride,
+                  void* partials, int64_t partial_rows, int n_gslots, void* stream);
 
 /* ---- tensor-network executor ----------------------------------------------- */
 /* desc: 4 + 9*32 int32 — bit counts of the index roles (h kept+shared, m kept

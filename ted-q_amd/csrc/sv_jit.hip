@@ -92,8 +92,8 @@ extern "C" int tedq_jit_unload(void* module) {
 // n <= 28).  Arguments as tedq_sv_apply_pass / tedq_sv_adjoint_pass; `fn` is
 // the generated kernel of this pass (same k, r and tile bits).
 extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, int n, int k, int r,
-                             uint64_t tile_mask, const void* mats, int64_t mat_stride, void* partials,
-                             int64_t partial_rows, int n_gslots, void* stream) {
+                             uint64_t tile_mask, uint64_t zero_mask, const void* mats, int64_t mat_stride,
+                             void* partials, int64_t partial_rows, int n_gslots, void* stream) {
   if (fn == nullptr || psi == nullptr || (adjoint && (lam == nullptr || (n_gslots > 0 && partials == nullptr))) ||
       batch <= 0 || n < 2 || n > 28 || k < 1 || k >= n || r < 1 || r > 5 || r > k || k - r > 9) {
     set_error("tedq_jit_pass: bad arguments (batch=%lld n=%d k=%d r=%d)", (long long)batch, n, k, r);
@@ -104,11 +104,15 @@ extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_
     set_error("tedq_jit_pass: tile mask does not have k=%d bits below n=%d", k, n);
     return kBadArgument;
   }
+  if ((zero_mask & (tile_mask | ~full)) != 0) {
+    set_error("tedq_jit_pass: zero mask 0x%llx holds tile bits or bits above n", (unsigned long long)zero_mask);
+    return kBadArgument;
+  }
   PassArgs a;
   a.mat_stride = mat_stride;
   a.batch = batch;
-  a.total_tiles = batch << (n - k);
-  a.rest_mask = full & ~tile_mask;
+  a.rest_mask = full & ~tile_mask & ~zero_mask;
+  a.total_tiles = batch << __builtin_popcountll(a.rest_mask);
   a.n = n;
   a.k = k;
   a.tT = k - r;

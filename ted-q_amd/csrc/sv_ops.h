@@ -37,8 +37,8 @@ constexpr int kStageWords = 48;
 struct PassArgs {
   int64_t mat_stride;
   int64_t batch;
-  int64_t total_tiles;      // batch * 2^(n-k)
-  uint64_t rest_mask;       // physical bits (< n) that are not tile bits
+  int64_t total_tiles;      // batch * 2^popcount(rest_mask)
+  uint64_t rest_mask;       // physical bits (< n) that are not tile bits and over which tiles run
   int n;
   int k;
   int tT;                   // log2(threads per tile)
@@ -186,8 +186,12 @@ __device__ __forceinline__ void bstore(float2 v, __amdgpu_buffer_rsrc_t r, uint3
 // share lines; swapping that bit with rest bit 3 makes those partners
 // blockIdx b and b^8, which land on the same XCD at about the same time, so
 // the second read can hit L2.
+// Tiles of a circuit are numbered over the bits of rest_mask (2^popcount of them: every non-tile bit,
+// or fewer when the caller leaves out bits on which the state vanishes).
+__device__ __forceinline__ int tiles_log2(const PassArgs& a) { return __popcll(a.rest_mask); }
+
 __device__ __forceinline__ void tile_origin(const PassArgs& a, int64_t tile, int64_t* circuit, int64_t* gbase) {
-  const int64_t tpc_log = a.n - a.k;
+  const int64_t tpc_log = tiles_log2(a);
   *circuit = tile >> tpc_log;
   uint64_t within = (uint64_t)(tile & ((1ll << tpc_log) - 1));
   if (tpc_log >= 4) within = (within & ~9ull) | ((within & 1ull) << 3) | ((within >> 3) & 1ull);

@@ -419,7 +419,7 @@ __device__ __forceinline__ void pass_adj_body(float2* __restrict__ psi, float2* 
   const int seg = tpt < 64 ? tpt : 64;       // lanes reduced together
   const int W = tpt >> 6 > 0 ? tpt >> 6 : 1; // partial rows per tile
   const int tl = tid & (tpt - 1);
-  int64_t tile_in_circuit = MULTI ? 0 : (blk & ((1ll << (a.n - a.k)) - 1));
+  int64_t tile_in_circuit = MULTI ? 0 : (blk & ((1ll << tiles_log2(a)) - 1));
   const int64_t prow = tile_in_circuit * W + (tl >> 6);
   const bool seg_writer = c.valid && ((tid & (seg - 1)) == 0);
   const bool wave_writer = (lane & 7) == 0;
@@ -630,8 +630,8 @@ static int launch_pass(bool adjoint, bool multi, dim3 grid, dim3 block, size_t l
 }
 
 static int run_pass(bool adjoint, void* psi, void* lam, int64_t batch, int n, int k, int r, int threads_log2,
-                    const int* dev_prog, uint64_t tile_mask, const void* mats, int64_t mat_stride,
-                    float* partials, int64_t np, int n_gslots, void* stream) {
+                    const int* dev_prog, uint64_t tile_mask, uint64_t zero_mask, const void* mats,
+                    int64_t mat_stride, float* partials, int64_t np, int n_gslots, void* stream) {
   if (psi == nullptr || (adjoint && (lam == nullptr || (n_gslots > 0 && partials == nullptr))) ||
       dev_prog == nullptr || batch <= 0 || n < 1 || n > 40 || k < 1 || k > n || r < 1 || r > 5 || r > k) {
     set_error("sv pass: bad arguments (batch=%lld n=%d k=%d r=%d)", (long long)batch, n, k, r);
@@ -648,11 +648,15 @@ static int run_pass(bool adjoint, void* psi, void* lam, int64_t batch, int n, in
     set_error("sv pass: tile mask does not have k=%d bits below n=%d", k, n);
     return kBadArgument;
   }
+  if ((zero_mask & (tile_mask | ~full)) != 0) {
+    set_error("sv pass: zero mask 0x%llx holds tile bits or bits above n", (unsigned long long)zero_mask);
+    return kBadArgument;
+  }
   PassArgs a;
   a.mat_stride = mat_stride;
   a.batch = batch;
-  a.total_tiles = batch << (n - k);
-  a.rest_mask = full & ~tile_mask;
+  a.rest_mask = full & ~tile_mask & ~zero_mask;
+  a.total_tiles = batch << __builtin_popcountll(a.rest_mask);
   a.n = n;
   a.k = k;
   a.tT = tT;
@@ -740,16 +744,16 @@ extern "C" int tedq_sv_pass_ragged(int adjoint, const void* dev_recs, int n_recs
 }
 
 extern "C" int tedq_sv_apply_pass(void* state, int64_t batch, int n_qubits, int tile_bits, int reg_bits,
-                                  int threads_log2, const int* dev_prog, uint64_t tile_mask, const void* mats,
-                                  int64_t mat_stride, void* stream) {
+                                  int threads_log2, const int* dev_prog, uint64_t tile_mask, uint64_t zero_mask,
+                                  const void* mats, int64_t mat_stride, void* stream) {
   return run_pass(false, state, nullptr, batch, n_qubits, tile_bits, reg_bits, threads_log2, dev_prog, tile_mask,
-                  mats, mat_stride, nullptr, 0, 0, stream);
+                  zero_mask, mats, mat_stride, nullptr, 0, 0, stream);
 }
 
 extern "C" int tedq_sv_adjoint_pass(void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits, int reg_bits,
-                                    int threads_log2, const int* dev_prog, uint64_t tile_mask, const void* mats,
-                                    int64_t mat_stride, float* partials, int64_t partial_rows, int n_gslots,
-                                    void* stream) {
-  return run_pass(true, psi, lam, batch, n_qubits, tile_bits, reg_bits, threads_log2, dev_prog, tile_mask, mats,
-                  mat_stride, partials, partial_rows, n_gslots, stream);
+                                    int threads_log2, const int* dev_prog, uint64_t tile_mask, uint64_t zero_mask,
+                                    const void* mats, int64_t mat_stride, float* partials, int64_t partial_rows,
+                                    int n_gslots, void* stream) {
+  return run_pass(true, psi, lam, batch, n_qubits, tile_bits, reg_bits, threads_log2, dev_prog, tile_mask,
+                  zero_mask, mats, mat_stride, partials, partial_rows, n_gslots, stream);
 }

@@ -22,9 +22,9 @@ _SIGNATURES = {
     "tedq_device_info": ([c_vp], c_int),
     "tedq_sv_build_tables": ([c_vp, c_int, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
                              c_int),
-    "tedq_sv_apply_pass": ([c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_u64, c_vp, c_i64, c_vp], c_int),
-    "tedq_sv_adjoint_pass": ([c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_u64, c_vp, c_i64, c_vp, c_i64,
-                              c_int, c_vp], c_int),
+    "tedq_sv_apply_pass": ([c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_u64, c_u64, c_vp, c_i64, c_vp], c_int),
+    "tedq_sv_adjoint_pass": ([c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_u64, c_u64, c_vp, c_i64, c_vp,
+                              c_i64, c_int, c_vp], c_int),
     "tedq_sv_pass_ragged": ([c_int, c_vp, c_int, c_i64, c_int, c_int, c_vp], c_int),
     "tedq_sv_init_zero": ([c_vp, c_i64, c_int, c_vp], c_int),
     "tedq_sv_init_copy": ([c_vp, c_vp, c_i64, c_int, c_vp], c_int),
@@ -44,8 +44,8 @@ _SIGNATURES = {
     "tedq_jit_compile": ([ctypes.c_char_p, c_int, c_vp, c_vp, c_vp], c_int),
     "tedq_jit_function": ([c_vp, ctypes.c_char_p, c_vp], c_int),
     "tedq_jit_unload": ([c_vp], c_int),
-    "tedq_jit_pass": ([c_vp, c_int, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_u64, c_vp, c_i64, c_vp, c_i64, c_int,
-                       c_vp], c_int),
+    "tedq_jit_pass": ([c_vp, c_int, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_u64, c_u64, c_vp, c_i64, c_vp, c_i64,
+                       c_int, c_vp], c_int),
     "tedq_tn_workspace_bytes": ([c_vp, c_i64], c_i64),
     "tedq_tn_contract": ([c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp], c_int),
     "tedq_tn_contract_small_batch": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_int, c_vp], c_int),

@@ -166,7 +166,7 @@ def pass_kernel_source(name, words, adjoint):
         L += ["  const __amdgpu_buffer_rsrc_t rl = make_rsrc(lam + gbase, (uint32_t)(8u << a.n));",
               f"  float2* ldl = lds + {T * R};",
               "  const int lane = tid & 63;",
-              f"  const int64_t prow = ((int64_t)blockIdx.x & ((1ll << (a.n - a.k)) - 1)) * {W} + (tid >> 6);",
+              f"  const int64_t prow = ((int64_t)blockIdx.x & ((1ll << tiles_log2(a)) - 1)) * {W} + (tid >> 6);",
               "  float* const pbase = partials + (circ * a.n_gslots * a.np + prow) * 8;",
               "  float sacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};",
               "  (void)lane; (void)sacc;"]

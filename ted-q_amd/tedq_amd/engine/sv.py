@@ -148,6 +148,24 @@ class _Program:
         self.enc = encode_plan(self.plan)
         self._dev = {}
         self._jit = {}          # device key -> JitModule of generated pass kernels
+        # From |0..0>, before pass p the state vanishes wherever a bit that no earlier pass held in its tile
+        # is 1, and pass p keeps the non-tile bits: its tiles with such a bit set hold zeros before and after
+        # it (forward), and in the adjoint sweep psi vanishes there and no earlier pass reads lambda there.
+        # zero_masks[p] (forward order) are those non-tile bits; the first pass runs one tile per circuit.
+        full = (1 << n) - 1
+        seen = 0
+        self.zero_masks = []
+        for mask in self.enc.tile_masks:
+            self.zero_masks.append(full & ~mask & ~seen if mask != full else 0)
+            seen |= mask
+        # gradient slots each pass writes (a contiguous range, assigned in pass order)
+        self.pass_gslots = []
+        for p in self.plan.passes:
+            gs = {self.plan.slots[op[4]].gslot for st in p.stages for op in st.ops
+                  if op[4] >= 0 and self.plan.slots[op[4]].gslot >= 0}
+            if gs and max(gs) - min(gs) + 1 != len(gs):
+                raise RuntimeError("gradient slots of a pass are not contiguous")
+            self.pass_gslots.append((min(gs), max(gs) + 1) if gs else (0, 0))
 
     def enable_jit(self, device, forward=True, adjoint=True):
         """Compile per-pass kernels for this program on ``device`` (hiprtc); returns True if any are in use.
@@ -215,7 +233,9 @@ class _Program:
             buf.err_checked = True
         return mats, amats
 
-    def forward_passes(self, psi, mats, probe=None):
+    def forward_passes(self, psi, mats, probe=None, from_zero=False):
+        """Apply the passes to psi in place.  ``from_zero``: psi holds |0..0> per row, so each pass skips the
+        tiles that hold zeros (zero_masks)."""
         dev = psi.device
         buf = self.buffers(dev)
         B = psi.shape[0]
@@ -223,18 +243,22 @@ class _Program:
         tl = self.threads_log2()
         fns = self._jit_fns(dev, False)
         for i, (off, mask) in enumerate(zip(self.enc.fwd_offsets, self.enc.tile_masks)):
+            zero = self.zero_masks[i] if from_zero else 0
             if probe:
                 probe.begin("sv_pass_fwd")
             if fns:
                 hip.call("tedq_jit_pass", fns[i], 0, hip.ptr(psi), None, B, self.n, self.plan.k, self.plan.r, mask,
-                         hip.ptr(mats), self.plan.mat_stride, None, 0, 0, st)
+                         zero, hip.ptr(mats), self.plan.mat_stride, None, 0, 0, st)
             else:
                 hip.call("tedq_sv_apply_pass", hip.ptr(psi), B, self.n, self.plan.k, self.plan.r, tl,
-                         buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, st)
+                         buf.pass_ptr(off), mask, zero, hip.ptr(mats), self.plan.mat_stride, st)
             if probe:
                 probe.end("sv_pass_fwd")
 
-    def adjoint_passes(self, psi, lam, mats, partials, probe=None):
+    def adjoint_passes(self, psi, lam, mats, partials, probe=None, from_zero=False):
+        """Reverse sweep over psi and lam (both in place), gradient partial rows into ``partials``.
+        ``from_zero``: the forward started from |0..0>, so each pass skips the tiles where psi vanishes;
+        the partial rows of skipped tiles are set to zero here."""
         dev = psi.device
         buf = self.buffers(dev)
         B = psi.shape[0]
@@ -243,16 +267,24 @@ class _Program:
         rows = self.partial_rows()
         G = self.plan.n_gslots
         fns = self._jit_fns(dev, True)
+        P = len(self.plan.passes)
+        if from_zero:
+            per_tile = rows >> (self.n - self.plan.k)
+            for zero, (g0, g1) in zip(self.zero_masks, self.pass_gslots):
+                if zero and g1 > g0:
+                    live = per_tile << (self.n - self.plan.k - bin(zero).count("1"))
+                    partials[:, g0:g1, live:, :].zero_()
         for i, (off, mask) in enumerate(zip(self.enc.adj_offsets, reversed(self.enc.tile_masks))):
+            zero = self.zero_masks[P - 1 - i] if from_zero else 0
             if probe:
                 probe.begin("sv_pass_adj")
             if fns:
                 hip.call("tedq_jit_pass", fns[i], 1, hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
-                         mask, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, st)
+                         mask, zero, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, st)
             else:
                 hip.call("tedq_sv_adjoint_pass", hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
-                         tl, buf.pass_ptr(off), mask, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows,
-                         G, st)
+                         tl, buf.pass_ptr(off), mask, zero, hip.ptr(mats), self.plan.mat_stride,
+                         hip.ptr(partials), rows, G, st)
             if probe:
                 probe.end("sv_pass_adj")
 
@@ -400,7 +432,7 @@ class SVExecutor:
             hip.call("tedq_sv_init_zero", hip.ptr(psi), B, self.n, st)
         else:
             hip.call("tedq_sv_init_copy", hip.ptr(psi), hip.ptr(init), B, self.n, st)
-        self.fwd.forward_passes(psi, mats, self.probe)
+        self.fwd.forward_passes(psi, mats, self.probe, from_zero=init is None)
         return psi
 
     def measure(self, psi):
@@ -580,7 +612,7 @@ class SVExecutor:
         G = prog.plan.n_gslots
         rows = prog.partial_rows()
         partials = torch.empty((B, max(G, 1), rows, 8), dtype=torch.float32, device=dev)
-        prog.adjoint_passes(psi, lam, mats, partials, self.probe)
+        prog.adjoint_passes(psi, lam, mats, partials, self.probe, from_zero=self.init_state is None)
         C = torch.empty((B, max(G, 1), 8), dtype=torch.float64, device=dev)
         buf = prog.buffers(dev)
         hip.call("tedq_sv_grad_reduce", hip.ptr(partials), B, G, rows, hip.ptr(amats), hip.ptr(buf.pgslot),

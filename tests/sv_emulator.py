@@ -192,8 +192,9 @@ def _apply_ops(regs_list, ops, mats, acc):
                 raise ValueError(f"unknown op {typ}")
 
 
-def run_pass(words, n, states, mats, acc):
-    """Execute one encoded pass over each state vector in ``states`` (in place)."""
+def run_pass(words, n, states, mats, acc, zero=0):
+    """Execute one encoded pass over each state vector in ``states`` (in place); tiles whose non-tile bits
+    meet ``zero`` are left untouched (the kernels' zero mask)."""
     k, r, nst = int(words[0]), int(words[1]), int(words[2])
     tile_bits = [int(b) for b in words[P.HDR_TILE_BITS:P.HDR_TILE_BITS + k]]
     rest = [b for b in range(n) if b not in tile_bits]
@@ -201,6 +202,8 @@ def run_pass(words, n, states, mats, acc):
     local_phys = np.array([_pdep(j, tile_bits) for j in range(1 << k)], dtype=np.int64)
     for t in range(1 << (n - k)):
         base = _pdep(t, rest)
+        if base & zero:
+            continue
         tiles = [s[base + local_phys].copy() for s in states]
         for si in range(nst):
             st = words[P.HDR_STAGES + P.STAGE_WORDS * si:P.HDR_STAGES + P.STAGE_WORDS * (si + 1)]
@@ -218,7 +221,7 @@ def run_pass(words, n, states, mats, acc):
             s[base + local_phys] = tile
 
 
-def forward(enc, plan, row, init=None):
+def forward(enc, plan, row, init=None, zero_masks=None):
     n = plan.n
     mats, amats = build_tables(enc, plan, row)
     psi = np.zeros(2 ** n, dtype=np.complex128)
@@ -226,16 +229,17 @@ def forward(enc, plan, row, init=None):
         psi[0] = 1
     else:
         psi[:] = np.asarray(init).reshape(-1)
-    for off in enc.fwd_offsets:
-        run_pass(enc.prog[off:], n, [psi], mats, {})
+    for i, off in enumerate(enc.fwd_offsets):
+        run_pass(enc.prog[off:], n, [psi], mats, {}, zero_masks[i] if zero_masks else 0)
     return psi, mats, amats
 
 
-def adjoint_grad(enc, plan, psi, lam, mats, amats, n_params):
+def adjoint_grad(enc, plan, psi, lam, mats, amats, n_params, zero_masks=None):
     acc = {}
     psi, lam = psi.copy(), lam.copy()
-    for off in enc.adj_offsets:
-        run_pass(enc.prog[off:], plan.n, [psi, lam], mats, acc)
+    P = len(enc.adj_offsets)
+    for i, off in enumerate(enc.adj_offsets):
+        run_pass(enc.prog[off:], plan.n, [psi, lam], mats, acc, zero_masks[P - 1 - i] if zero_masks else 0)
     grad = np.zeros(n_params)
     for p in range(n_params):
         g = int(enc.pgslot[p])

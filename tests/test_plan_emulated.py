@@ -337,3 +337,34 @@ def test_fold_keeps_everything_for_non_z_measurements():
     # last layer: RY on qubit 0; layer 2: rotations on qubits 0-1 and CNOT(0,1); layer 1: rotations on
     # qubits 0-2 and CNOT(0,1), CNOT(1,2)
     assert ex.n == 3 and len(ex._units) == 1 + (4 + 1) + (6 + 2)
+
+
+@pytest.mark.parametrize("seed,n,k,r", [(30, 7, 4, 2), (31, 8, 5, 3), (32, 9, 5, 2)])
+def test_passes_skip_tiles_that_hold_zeros(seed, n, k, r):
+    """From |0..0>, pass p only needs the tiles whose non-tile bits lie in earlier passes' tiles (zero
+    masks of the executor's programs): skipping the others gives the same state, adjoint gradients and
+    uncomputed |0..0> (up to the skipped tiles, which hold zeros) as running every tile."""
+    from tedq_amd.engine.sv import _Program
+    rng = np.random.default_rng(seed)
+    spec = random_spec(n, 40, rng)
+    npar = sum(s[2] for s in spec)
+    values = [float(x) for x in rng.uniform(-math.pi, math.pi, npar)]
+    circ = trace(n, spec, values, lambda: qai.expval(qai.PauliZ(qubits=[0])))
+    nq, units, P, pool = lower_circuit(circ.compilecircuit(backend="pytorch"))
+    for specialize in (False, True):
+        prog = _Program(nq, units, P, pool, k, r, specialize=specialize)
+        zm = prog.zero_masks
+        assert zm[0] == ((1 << n) - 1) & ~prog.enc.tile_masks[0]
+        row = np.asarray(values)
+        psi_full, mats, amats = EMU.forward(prog.enc, prog.plan, row)
+        psi_skip, _, _ = EMU.forward(prog.enc, prog.plan, row, zero_masks=zm)
+        assert np.allclose(psi_skip, psi_full, atol=1e-12)
+        lam = (1 - 2 * ((np.arange(2 ** n) >> (n - 1)) & 1)) * psi_full
+        g_full, _ = EMU.adjoint_grad(prog.enc, prog.plan, psi_full, lam, mats, amats, P)
+        g_skip, _ = EMU.adjoint_grad(prog.enc, prog.plan, psi_full, lam, mats, amats, P, zero_masks=zm)
+        assert np.allclose(g_skip, g_full, atol=1e-10)
+        # each pass's gradient slots form one contiguous range
+        for (g0, g1), p in zip(prog.pass_gslots, prog.plan.passes):
+            used = {prog.plan.slots[op[4]].gslot for st in p.stages for op in st.ops
+                    if op[4] >= 0 and prog.plan.slots[op[4]].gslot >= 0}
+            assert used == set(range(g0, g1))
