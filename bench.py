@@ -338,17 +338,20 @@ def serving_leg(n_req, warm, ragged=True):
 
 
 def measured_traffic(kernel, n, layers, batch, ex):
-    """HBM bytes per launch of ``kernel`` from the committed FETCH_SIZE / WRITE_SIZE counter runs of this
-    workload (profiles/sv_pass_traffic_hea20_b256.json), when the run uses the profiled shapes; else None."""
+    """Mean HBM bytes per launch of ``kernel`` from the committed FETCH_SIZE / WRITE_SIZE counter runs of
+    this workload (profiles/sv_pass_traffic_hea20_b256.json: one record per profiled adjoint shape), when
+    the run uses a profiled shape; else None."""
     path = os.path.join(ROOT, "profiles", "sv_pass_traffic_hea20_b256.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        rec = json.load(f)
-    w = rec["workload"]
-    same = (rec["kernel"] == kernel and w["qubits"] == n and w["layers"] == layers and w["batch"] == batch
-            and w["tile_bits_bwd"] == ex.bwd.plan.k and w["reg_bits_bwd"] == ex.bwd.plan.r)
-    return rec["traffic_bytes_per_launch"] if same else None
+        recs = json.load(f)
+    for rec in recs if isinstance(recs, list) else [recs]:
+        w = rec["workload"]
+        if (rec["kernel"] == kernel and w["qubits"] == n and w["layers"] == layers and w["batch"] == batch
+                and w["tile_bits_bwd"] == ex.bwd.plan.k and w["reg_bits_bwd"] == ex.bwd.plan.r):
+            return rec["traffic_bytes_per_launch"]
+    return None
 
 
 def tn_leg(c4=True, c3=True):
