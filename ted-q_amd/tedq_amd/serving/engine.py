@@ -8,10 +8,13 @@ the circuit's measurements and the full parameter gradient.  The loop
 * queues requests per key as they arrive and forms batches of structurally
   identical circuits from whatever is waiting when a slot frees up, oldest
   request first;
-* with ``ragged=True`` queues by tile shape instead: a batch may mix circuit
-  structures (depths, qubit counts from 12 up), and every pass index of all
-  of them runs as one ragged launch (tedq_amd/engine/ragged.py), while a batch
-  that happens to hold one structure takes the same-structure path;
+* with ``ragged=True`` queues by tile shape instead: once ``ragged_min``
+  requests of a tile shape wait, a batch may mix circuit structures (depths
+  and qubit counts), and every pass index of all of them runs as one ragged
+  launch (tedq_amd/engine/ragged.py); with fewer waiting (light load), or when
+  the oldest requests share one structure, the oldest request's structure is
+  served at once on the same-structure path (captured graphs), which costs the
+  host far less per batch;
 * admits a batch only while the bytes of all in-flight batches stay within an
   HBM budget (requests wait otherwise);
 * runs batches on several HIP streams so their kernels overlap on the device,
@@ -111,10 +114,11 @@ class ServingEngine:
     """
 
     def __init__(self, factory, device=None, hbm_budget=32 << 30, max_batch=64, n_streams=4, max_inflight=8,
-                 graph_max_qubits=18, ragged=False):
+                 graph_max_qubits=18, ragged=False, ragged_min=8):
         self.factory = factory
         # queue by tile shape and mix structures in one launch sequence (see module docstring)
         self.ragged = ragged
+        self.ragged_min = ragged_min
         self.ragged_batches = 0
         self.graph_max_qubits = graph_max_qubits
         self._graphs = {}
@@ -219,11 +223,23 @@ class ServingEngine:
         q = self._queues[qkey]
         if self.ragged:
             head = [q[i].key for i in range(min(len(q), self.max_batch))]
-            if len(set(head)) > 1:
+            if len(set(head)) > 1 and len(q) >= self.ragged_min:
                 return self._launch_ragged(qkey, now)
             key = head[0]
+            if len(set(head)) > 1:
+                # light load: the oldest request's structure at once (with its twins)
+                mine = deque(r for r in q if r.key == key)
+                others = [r for r in q if r.key != key]
+                launched = self._launch_same(key, mine, now)
+                if launched:
+                    self._queues[qkey] = deque(sorted(list(mine) + others, key=lambda r: r.arrival))
+                return launched
         else:
             key = qkey
+        return self._launch_same(key, q, now)
+
+    def _launch_same(self, key, q, now):
+        """A batch of the structure ``key`` from the head of ``q`` (requests of that structure only)."""
         ex = self.executor(key)
         if ex.n <= self.graph_max_qubits and self._launch_graph(key, ex, now, q):
             return True

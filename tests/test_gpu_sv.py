@@ -711,3 +711,25 @@ def test_second_backward_through_the_same_graph():
     g1 = torch.autograd.grad(out.sum(), rows, retain_graph=True)[0]
     g2 = torch.autograd.grad(out.sum(), rows)[0]
     assert torch.equal(g1, g2)
+
+
+def test_circuit_that_folds_away_entirely():
+    """Every gate folds into the observables (RZ, CNOT from a |0> control, PhaseShift before <Z>): no pass
+    is left, the register shrinks to one qubit, values are 1 and gradients 0, as in the oracle."""
+    def cd(a, b):
+        qai.RZ(a, qubits=[0])
+        qai.CNOT(qubits=[0, 1])
+        qai.PhaseShift(b, qubits=[1])
+        return [qai.expval(qai.PauliZ(qubits=[1])), qai.expval(qai.PauliZ(qubits=[0]))]
+    c = qai.Circuit(cd, 2, 0.3, 0.4).compilecircuit(backend="pytorch")
+    ex = c.executor()
+    assert ex.n == 1 and not ex.fwd.plan.passes
+    rows = torch.tensor([[0.3, 0.4], [1.1, -2.0]], dtype=torch.float32, device=DEV, requires_grad=True)
+    out = c.execute_batch(rows)
+    out.sum().backward()
+    t64 = torch.tensor([1.1, -2.0], dtype=torch.float64, requires_grad=True)
+    ref = O.execute(2, [("RZ", [0], [t64[0]]), ("CNOT", [0, 1], []), ("PhaseShift", [1], [t64[1]])],
+                    [("expval", ("PauliZ", [1])), ("expval", ("PauliZ", [0]))])
+    ref.sum().backward()
+    assert np.allclose(out[1].detach().cpu().numpy(), ref.detach().numpy(), atol=1e-6)
+    assert np.allclose(rows.grad[1].cpu().numpy(), t64.grad.numpy(), atol=1e-6)
