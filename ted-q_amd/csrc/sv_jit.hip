@@ -93,7 +93,8 @@ extern "C" int tedq_jit_unload(void* module) {
 // the generated kernel of this pass (same k, r and tile bits).
 extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, int n, int k, int r,
                              uint64_t tile_mask, uint64_t zero_mask, const void* mats, int64_t mat_stride,
-                             void* partials, int64_t partial_rows, int n_gslots, void* stream) {
+                             void* partials, int64_t partial_rows, int n_gslots, const double* seed,
+                             void* stream) {
   if (fn == nullptr || psi == nullptr || (adjoint && (lam == nullptr || (n_gslots > 0 && partials == nullptr))) ||
       batch <= 0 || n < 2 || n > 28 || k < 1 || k >= n || r < 1 || r > 5 || r > k || k - r > 9) {
     set_error("tedq_jit_pass: bad arguments (batch=%lld n=%d k=%d r=%d)", (long long)batch, n, k, r);
@@ -125,7 +126,7 @@ extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_
     set_error("tedq_jit_pass: tile of %zu LDS bytes or %lld tiles is out of range", lds, (long long)a.total_tiles);
     return kBadArgument;
   }
-  void* args[] = {&psi, &lam, const_cast<void**>(&mats), &partials, &a};
+  void* args[] = {&psi, &lam, const_cast<void**>(&mats), &partials, &a, const_cast<double**>(&seed)};
   const hipError_t e = hipModuleLaunchKernel(static_cast<hipFunction_t>(fn), (unsigned)a.total_tiles, 1, 1, T, 1, 1,
                                              (unsigned)lds, static_cast<hipStream_t>(stream), args, nullptr);
   if (e != hipSuccess) {

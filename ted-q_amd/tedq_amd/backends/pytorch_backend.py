@@ -94,8 +94,7 @@ class _AdjointFunction(torch.autograd.Function):
         (rows,) = ctx.saved_tensors
         psi = ctx.psi if ctx.psi is not None else ex.run_forward(rows)
         ctx.psi = None
-        lam = ex.seed_adjoint(psi, grads)
-        grad = ex.run_backward(rows, psi, lam, grad_dtype=rows.dtype)
+        grad = ex.adjoint(rows, psi, grads, grad_dtype=rows.dtype)
         return None, grad, None
 
 

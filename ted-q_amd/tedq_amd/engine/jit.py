@@ -141,8 +141,31 @@ def _op_lines(w, R, regs, adjoint, seg):
     return out
 
 
-def pass_kernel_source(name, words, adjoint):
-    """HIP source of one generated pass kernel from its program words (program.py layout)."""
+def _seed_lines(hi, R, masks):
+    """Per-register lambda / psi factors d_m = sum_j c_j (-1)^popc(x & mask_j) for x = rest bits | lane
+    bits | register m's bits: the rest and lane parities per thread, the register parities as constants."""
+    out = ["  const double* __restrict__ cb = seed + circ * %d;" % len(masks),
+           "  const uint64_t xr = (uint64_t)(gbase - (circ << a.n)) | (uint64_t)lo;"]
+    for j, mk in enumerate(masks):
+        out.append(f"  const float t{j} = (__popcll(xr & {mk}ull) & 1) ? -(float)cb[{j}] : (float)cb[{j}];")
+    for m in range(R):
+        terms = []
+        for j, mk in enumerate(masks):
+            neg = bin(int(hi[m]) & mk).count("1") & 1
+            terms.append(("- " if neg else "+ ") + f"t{j}")
+        expr = " ".join(terms).lstrip("+ ") if terms else "0.f"
+        if expr.startswith("- "):
+            expr = "-" + expr[2:]
+        out.append(f"  const float d{m} = {expr};")
+    return out
+
+
+def pass_kernel_source(name, words, adjoint, seed_masks=None):
+    """HIP source of one generated pass kernel from its program words (program.py layout).
+
+    ``seed_masks`` (adjoint only): instead of loading lambda, form it from the loaded psi as
+    lambda(x) = sum_j seed[circuit][j] (-1)^popc(x & mask_j) psi(x) -- the adjoint seed of Z-string
+    expectation values -- for the first pass of the sweep."""
     words = np.asarray(words, dtype=np.int32)
     k, r, nst, tT = (int(x) for x in words[:4])
     R, T = 1 << r, 1 << tT
@@ -152,7 +175,7 @@ def pass_kernel_source(name, words, adjoint):
     W = max(T >> 6, 1)
     L = [f'extern "C" __global__ void __launch_bounds__({T}) {name}(',
          "    float2* __restrict__ psi, float2* __restrict__ lam, const float2* __restrict__ mats,",
-         "    float* __restrict__ partials, tedq::PassArgs a) {",
+         "    float* __restrict__ partials, tedq::PassArgs a, const double* __restrict__ seed) {",
          "  using namespace tedq;",
          "  extern __shared__ float2 lds[];",
          "  const int tid = threadIdx.x;",
@@ -170,7 +193,14 @@ def pass_kernel_source(name, words, adjoint):
               "  float* const pbase = partials + (circ * a.n_gslots * a.np + prow) * 8;",
               "  float sacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};",
               "  (void)lane; (void)sacc;"]
-    for m in range(R):
+    if adjoint and seed_masks is not None:
+        L += _seed_lines(hi, R, seed_masks)
+        for m in range(R):
+            x = lds_swizzle(m * T)
+            L.append(f"  {{ const float2 v = bload(rp, lo * 8u, {int(hi[m]) * 8}u); lds[sw ^ {x}] = v;"
+                     f" ldl[sw ^ {x}] = make_float2(d{m} * v.x, d{m} * v.y); }}")
+    L.append("  (void)seed;")
+    for m in range(R if seed_masks is None or not adjoint else 0):
         L.append(f"  lds[sw ^ {lds_swizzle(m * T)}] = bload(rp, lo * 8u, {int(hi[m]) * 8}u);")
         if adjoint:
             L.append(f"  ldl[sw ^ {lds_swizzle(m * T)}] = bload(rl, lo * 8u, {int(hi[m]) * 8}u);")
@@ -203,8 +233,9 @@ def pass_kernel_source(name, words, adjoint):
     return "\n".join(L)
 
 
-def program_source(enc, n_fwd, n_adj):
-    """Generated source of every pass of an encoded plan: kernels f0.. (forward) and a0.. (adjoint)."""
+def program_source(enc, n_fwd, n_adj, seed_masks=None):
+    """Generated source of every pass of an encoded plan: kernels f0.. (forward) and a0.. (adjoint), and
+    with ``seed_masks`` also a0s, the first adjoint pass forming lambda from psi (pass_kernel_source)."""
     parts = ['#include "sv_ops.h"', ""]
     offs = list(enc.fwd_offsets) + list(enc.adj_offsets) + [len(enc.prog)]
     starts = sorted(set(offs))
@@ -215,6 +246,9 @@ def program_source(enc, n_fwd, n_adj):
     for i in range(n_adj):
         o = enc.adj_offsets[i]
         parts.append(pass_kernel_source(f"a{i}", enc.prog[o:ends[o]], True))
+    if seed_masks is not None and n_adj:
+        o = enc.adj_offsets[0]
+        parts.append(pass_kernel_source("a0s", enc.prog[o:ends[o]], True, seed_masks))
     return "\n\n".join(parts) + "\n"
 
 

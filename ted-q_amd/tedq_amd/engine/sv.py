@@ -167,27 +167,39 @@ class _Program:
                 raise RuntimeError("gradient slots of a pass are not contiguous")
             self.pass_gslots.append((min(gs), max(gs) + 1) if gs else (0, 0))
 
-    def enable_jit(self, device, forward=True, adjoint=True):
+    def enable_jit(self, device, forward=True, adjoint=True, seed_masks=None):
         """Compile per-pass kernels for this program on ``device`` (hiprtc); returns True if any are in use.
 
         Forward and adjoint kernels are compiled separately, once each; modules stay loaded for the
-        life of the program (kernels may still be running when a program is dropped).
+        life of the program (kernels may still be running when a program is dropped).  ``seed_masks``:
+        the Z-string masks of an executor whose outputs are all Z-string expectations; the adjoint
+        module then also holds a first pass that forms lambda from psi (jit.pass_kernel_source).
         """
         from . import jit  # pylint: disable=import-outside-toplevel
         key = (device.type, device.index)
         k, r = self.plan.k, self.plan.r
-        have = self._jit.setdefault(key, {"f": None, "a": None, "mods": []})
+        have = self._jit.setdefault(key, {"f": None, "a": None, "a_seed": None, "mods": []})
         for kind, want, adj in (("f", forward, False), ("a", adjoint, True)):
             if not want or have[kind] is not None or not jit.supported(self.n, k, r, adj):
                 continue
             offs = self.enc.adj_offsets if adj else self.enc.fwd_offsets
             names = [f"{kind}{i}" for i in range(len(offs))]
-            src = jit.program_source(self.enc, 0 if adj else len(offs), len(offs) if adj else 0)
+            masks = seed_masks if adj and offs else None
+            src = jit.program_source(self.enc, 0 if adj else len(offs), len(offs) if adj else 0, masks)
             with torch.cuda.device(device):
-                mod = jit.JitModule(src, names)
+                mod = jit.JitModule(src, names + (["a0s"] if masks is not None else []))
             have["mods"].append(mod)
             have[kind] = [mod.functions[nm] for nm in names]
+            if masks is not None:
+                have["a_seed"] = (tuple(masks), mod.functions["a0s"])
         return have["f"] is not None or have["a"] is not None
+
+    def seed_fn(self, device, masks):
+        """The generated first adjoint pass that forms lambda from psi for these Z-string masks, or None."""
+        got = self._jit.get((device.type, device.index))
+        if got is None or got["a_seed"] is None or got["a_seed"][0] != tuple(masks):
+            return None
+        return got["a_seed"][1]
 
     def _jit_fns(self, device, adjoint):
         got = self._jit.get((device.type, device.index))
@@ -248,17 +260,18 @@ class _Program:
                 probe.begin("sv_pass_fwd")
             if fns:
                 hip.call("tedq_jit_pass", fns[i], 0, hip.ptr(psi), None, B, self.n, self.plan.k, self.plan.r, mask,
-                         zero, hip.ptr(mats), self.plan.mat_stride, None, 0, 0, st)
+                         zero, hip.ptr(mats), self.plan.mat_stride, None, 0, 0, None, st)
             else:
                 hip.call("tedq_sv_apply_pass", hip.ptr(psi), B, self.n, self.plan.k, self.plan.r, tl,
                          buf.pass_ptr(off), mask, zero, hip.ptr(mats), self.plan.mat_stride, st)
             if probe:
                 probe.end("sv_pass_fwd")
 
-    def adjoint_passes(self, psi, lam, mats, partials, probe=None, from_zero=False):
+    def adjoint_passes(self, psi, lam, mats, partials, probe=None, from_zero=False, seed=None):
         """Reverse sweep over psi and lam (both in place), gradient partial rows into ``partials``.
         ``from_zero``: the forward started from |0..0>, so each pass skips the tiles where psi vanishes;
-        the partial rows of skipped tiles are set to zero here."""
+        the partial rows of skipped tiles are set to zero here.  ``seed`` = (kernel, dy [B, M] float64):
+        the first pass forms lambda from psi with that generated kernel (lam is only written)."""
         dev = psi.device
         buf = self.buffers(dev)
         B = psi.shape[0]
@@ -267,6 +280,8 @@ class _Program:
         rows = self.partial_rows()
         G = self.plan.n_gslots
         fns = self._jit_fns(dev, True)
+        if seed is not None and not fns:
+            raise RuntimeError("a fused adjoint seed needs the generated adjoint kernels")
         P = len(self.plan.passes)
         if from_zero:
             per_tile = rows >> (self.n - self.plan.k)
@@ -279,8 +294,9 @@ class _Program:
             if probe:
                 probe.begin("sv_pass_adj")
             if fns:
-                hip.call("tedq_jit_pass", fns[i], 1, hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
-                         mask, zero, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, st)
+                fn, dy = (seed[0], hip.ptr(seed[1])) if seed is not None and i == 0 else (fns[i], None)
+                hip.call("tedq_jit_pass", fn, 1, hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
+                         mask, zero, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, dy, st)
             else:
                 hip.call("tedq_sv_adjoint_pass", hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
                          tl, buf.pass_ptr(off), mask, zero, hip.ptr(mats), self.plan.mat_stride,
@@ -359,9 +375,16 @@ class SVExecutor:
             k, r = self.bwd.plan.k, self.bwd.plan.r
             if jit.supported(self.n, k, r, True):
                 self.bwd = self._program(k, r, specialize=True)
-            self.bwd.enable_jit(device, False, True)
+            self.bwd.enable_jit(device, False, True, self._seed_masks())
         # pylint: disable=protected-access
         return self.fwd._jit_fns(device, False) is not None, self.bwd._jit_fns(device, True) is not None
+
+    def _seed_masks(self):
+        """Z-string masks whose adjoint seed a generated first pass may form (every output a Z-string
+        expectation, at most ZMAX of them), else None."""
+        if self.specs and all(s.kind == "zexp" for s in self.specs) and 0 < len(self.zmasks) <= ZMAX:
+            return tuple(self.zmasks)
+        return None
 
     def _ensure_jit(self, device):
         key = (device.type, device.index)
@@ -539,6 +562,37 @@ class SVExecutor:
         return out
 
     # ------------------------------------------------------------------
+    def _zexp_columns(self, grads, B, dev):
+        """Upstream gradient of every Z-string mask, [B] float64 each (summed over the outputs sharing a
+        mask, zeros where none)."""
+        cols = [None] * len(self.zmasks)
+        for spec, g in zip(self.specs, grads):
+            if spec.kind == "zexp" and g is not None:
+                g = g.to(torch.float64).reshape(B)
+                cols[spec.zindex] = g if cols[spec.zindex] is None else cols[spec.zindex] + g
+        zero = None
+        for i, c in enumerate(cols):
+            if c is None:
+                if zero is None:
+                    zero = torch.full((B,), 0.0, dtype=torch.float64, device=dev)
+                cols[i] = zero
+        return cols
+
+    def adjoint(self, params, psi, grads, grad_dtype=torch.float32):
+        """Seed lambda from the upstream gradients (aligned with self.specs) and run the adjoint sweep;
+        psi is consumed.  Returns grad [B, P].  When every output is a Z-string expectation and generated
+        kernels run the sweep, its first pass forms lambda from psi itself (no separate seed kernel)."""
+        dev = psi.device
+        if not self.n_params:
+            return torch.empty((psi.shape[0], 0), dtype=grad_dtype, device=dev)
+        self._ensure_jit(dev)
+        masks = self._seed_masks()
+        fn = self.bwd.seed_fn(dev, masks) if masks is not None else None
+        if fn is not None:
+            dy = torch.stack(self._zexp_columns(grads, psi.shape[0], dev), 1).contiguous()
+            return self.run_backward(params, psi, torch.empty_like(psi), grad_dtype, seed=(fn, dy))
+        return self.run_backward(params, psi, self.seed_adjoint(psi, grads), grad_dtype)
+
     def seed_adjoint(self, psi, grads):
         """lambda = sum_m (d out_m)^* seed_m applied to psi; grads aligned with self.specs (None = zero)."""
         dev = psi.device
@@ -550,17 +604,7 @@ class SVExecutor:
         dy = None
         cols = []
         if M:
-            cols = [None] * M
-            for spec, g in zip(self.specs, grads):
-                if spec.kind == "zexp" and g is not None:
-                    g = g.to(torch.float64).reshape(B)
-                    cols[spec.zindex] = g if cols[spec.zindex] is None else cols[spec.zindex] + g
-            zero = None
-            for i, c in enumerate(cols):
-                if c is None:
-                    if zero is None:
-                        zero = torch.full((B,), 0.0, dtype=torch.float64, device=dev)
-                    cols[i] = zero
+            cols = self._zexp_columns(grads, B, dev)
             dy = torch.stack(cols[:ZMAX], 1).contiguous()
         m0 = min(M, ZMAX)
         hip.call("tedq_sv_lambda_z", hip.ptr(psi), hip.ptr(lam), B, self.n, hip.ptr(zbuf) if m0 else None, m0,
@@ -599,8 +643,9 @@ class SVExecutor:
                 hip.call("tedq_sv_axpy", hip.ptr(lam), hip.ptr(gs), B, self.n, 0.5, st)
         return lam
 
-    def run_backward(self, params, psi, lam, grad_dtype=torch.float32):
-        """Adjoint sweep for [B, P] rows; consumes psi and lam (both are overwritten).  Returns grad [B, P]."""
+    def run_backward(self, params, psi, lam, grad_dtype=torch.float32, seed=None):
+        """Adjoint sweep for [B, P] rows; consumes psi and lam (both are overwritten).  Returns grad [B, P].
+        ``seed``: (kernel, dy) for a first pass that forms lambda from psi (see ``adjoint``)."""
         dev = psi.device
         B = psi.shape[0]
         grad = torch.empty((B, max(self.n_params, 1)), dtype=grad_dtype, device=dev)
@@ -612,7 +657,7 @@ class SVExecutor:
         G = prog.plan.n_gslots
         rows = prog.partial_rows()
         partials = torch.empty((B, max(G, 1), rows, 8), dtype=torch.float32, device=dev)
-        prog.adjoint_passes(psi, lam, mats, partials, self.probe, from_zero=self.init_state is None)
+        prog.adjoint_passes(psi, lam, mats, partials, self.probe, from_zero=self.init_state is None, seed=seed)
         C = torch.empty((B, max(G, 1), 8), dtype=torch.float64, device=dev)
         buf = prog.buffers(dev)
         hip.call("tedq_sv_grad_reduce", hip.ptr(partials), B, G, rows, hip.ptr(amats), hip.ptr(buf.pgslot),
@@ -628,8 +673,7 @@ class SVExecutor:
         """
         psi = self.run_forward(rows)
         outs = self.measure(psi)
-        lam = self.seed_adjoint(psi, grads_out)
-        grad = self.run_backward(rows, psi, lam, grad_dtype=rows.dtype)
+        grad = self.adjoint(rows, psi, grads_out, grad_dtype=rows.dtype)
         return outs, grad
 
     def memory_bytes(self, batch, with_grad=True):
