@@ -10,9 +10,10 @@ inputs are resident in HBM when the timed region starts.
 Extra fields on the JSON line:
 * roofline: the dominant pass kernel, algorithmic bytes per launch (16 B per
   amplitude for a forward pass, 32 B for an adjoint pass over psi and lambda,
-  over the tiles the pass processes: from |0..0> the first passes skip the
-  tiles that hold zeros) divided by its mean launch time, measured with events
-  on the launch stream during the timed steps, against 8 TB/s;
+  24 B when it reads psi from the forward's checkpoints, over the tiles the pass
+  processes: from |0..0> the first passes skip the tiles that hold zeros)
+  divided by its mean launch time, measured with events on the launch stream
+  during the timed steps, against 8 TB/s;
 * cpu_baseline: the complex64 CPU restatement of the reference algorithm
   (tensordot + permute per gate, torch autograd backward) on the host cores,
   on a bounded sample of the same workload;
@@ -529,7 +530,9 @@ def main():
         from tedq_amd.engine.autotune import autotune  # pylint: disable=import-outside-toplevel
         res = autotune(ex, rows, jit=not args.no_jit, log=progress)
         tuning = {"best": list(res["best"]), "fwd_ms": {f"{k}/{r}": v for (k, r), v in res["fwd"].items()},
-                  "bwd_ms": {f"{k}/{r}": v for (k, r), v in res["bwd"].items()}}
+                  "bwd_ms": {f"{k}/{r}": v for (k, r), v in res["bwd"].items()},
+                  "checkpointed_fwd_plus_bwd_ms": {f"{a}/{b}+{c}/{d}": v
+                                                   for (a, b, c, d), v in res["checkpointed"].items()}}
 
     jit_s = None
     if not args.no_jit:
@@ -569,7 +572,9 @@ def main():
     # that hold zeros (zero masks), so their launches move a fraction of the state
     live = {"sv_pass_fwd": [2.0 ** -bin(z).count("1") for z in ex.fwd.zero_masks],
             "sv_pass_adj": [2.0 ** -bin(z).count("1") for z in ex.bwd.zero_masks]}
-    for kind, bpa in (("sv_pass_fwd", 16), ("sv_pass_adj", 32)):
+    # with per-pass checkpoints the adjoint reads psi and lambda and writes lambda only (24 B per amplitude)
+    ckpt = ex._checkpoints_fit(dev, B, True)  # pylint: disable=protected-access
+    for kind, bpa in (("sv_pass_fwd", 16), ("sv_pass_adj", 24 if ckpt else 32)):
         ms, cnt = probe.mean_ms(kind)
         if ms:
             frac = float(np.mean(live[kind]))
@@ -592,6 +597,7 @@ def main():
                    # exact simplifications: the last RZ layer and CNOT chain folded into the Z-string
                    # observables; diagonal gates applied without their global phase (expectations only)
                    "folded_gates": ex.folded, "phase_free_forward": ex.fwd.plan.specialize,
+                   "forward_checkpoints": ckpt,
                    "pass_kernels": "interpreted" if args.no_jit else
                    {"generated_fwd": jit_used[0], "generated_bwd": jit_used[1], "compile_s": jit_s}},
         "autotune": tuning,

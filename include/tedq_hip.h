@@ -170,10 +170,14 @@ int tedq_jit_unload(void* module);
  * arguments as tedq_sv_apply_pass / tedq_sv_adjoint_pass (k < n <= 28, at most
  * 64 KiB of LDS per workgroup).  seed: dev double [batch][M], read only by a
  * generated first adjoint pass that forms lambda from psi (the Z-string
- * adjoint seed, tedq_sv_lambda_z, fused into the pass); null otherwise. */
+ * adjoint seed, tedq_sv_lambda_z, fused into the pass); null otherwise.
+ * flags: 1 (forward) reads psi and writes the result to lam, a separate state
+ * buffer (the forward keeps every pass's output as a checkpoint); 2 (adjoint)
+ * psi is the checkpoint of the forward state at the end of this pass and is
+ * only read (the state at its start is the previous checkpoint).  0: in place. */
 int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits,
                   int reg_bits, uint64_t tile_mask, uint64_t zero_mask, const void* mats, int64_t mat_stride,
-                  void* partials, int64_t partial_rows, int n_gslots, const double* seed, void* stream);
+                  void* partials, int64_t partial_rows, int n_gslots, const double* seed, int flags, void* stream);
 
 /* ---- tensor-network executor ----------------------------------------------- */
 /* desc: 4 + 9*32 int32 — bit counts of the index roles (h kept+shared, m kept

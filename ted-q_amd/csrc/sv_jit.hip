@@ -93,7 +93,7 @@ extern "C" int tedq_jit_unload(void* module) {
 // the generated kernel of this pass (same k, r and tile bits).
 extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, int n, int k, int r,
                              uint64_t tile_mask, uint64_t zero_mask, const void* mats, int64_t mat_stride,
-                             void* partials, int64_t partial_rows, int n_gslots, const double* seed,
+                             void* partials, int64_t partial_rows, int n_gslots, const double* seed, int flags,
                              void* stream) {
   if (fn == nullptr || psi == nullptr || (adjoint && (lam == nullptr || (n_gslots > 0 && partials == nullptr))) ||
       batch <= 0 || n < 2 || n > 28 || k < 1 || k >= n || r < 1 || r > 5 || r > k || k - r > 9) {
@@ -120,6 +120,12 @@ extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_
   a.gT = 0;
   a.np = partial_rows;
   a.n_gslots = n_gslots;
+  a.flags = flags;
+  if ((flags & ~(kPassOutOfPlace | kPassPsiReadOnly)) != 0 || (!adjoint && (flags & kPassOutOfPlace) && lam == nullptr) ||
+      (adjoint && (flags & kPassOutOfPlace)) || (!adjoint && (flags & kPassPsiReadOnly))) {
+    set_error("tedq_jit_pass: flags %d do not fit a %s pass", flags, adjoint ? "adjoint" : "forward");
+    return kBadArgument;
+  }
   const unsigned T = 1u << (k - r);
   const size_t lds = (size_t)(1u << k) * sizeof(float2) * (adjoint ? 2 : 1);
   if (lds > 65536 || a.total_tiles > 0x7fffffffll) {

@@ -43,9 +43,16 @@ struct PassArgs {
   int k;
   int tT;                   // log2(threads per tile)
   int gT;                   // log2(tiles per workgroup)
-  int64_t np;               // adjoint: partials are [batch][n_gslots][np][8]               // partial rows per gradient slot per circuit
+  int64_t np;               // adjoint: partials are [batch][n_gslots][np][8] (partial rows per slot per circuit)
   int n_gslots;
+  int flags;                // generated passes: kPassOutOfPlace, kPassPsiReadOnly
 };
+
+// Generated forward pass: read psi, write the result to the second state pointer (a checkpoint).
+constexpr int kPassOutOfPlace = 1;
+// Generated adjoint pass: psi holds the forward state at the end of this pass (a checkpoint) and is only
+// read; the state at its start is the previous checkpoint, so psi is not written back.
+constexpr int kPassPsiReadOnly = 2;
 
 // ---------------------------------------------------------------------------
 // register-level gate kernels.  B is the target register bit and CM a control

@@ -569,6 +569,7 @@ __device__ __forceinline__ PassArgs ragged_args(const RaggedRec& rc, int k, int 
   const int W = tT > 6 ? 1 << (tT - 6) : 1;
   a.np = (1ll << (rc.n - k)) * W;
   a.n_gslots = rc.n_gslots;
+  a.flags = 0;
   return a;
 }
 
@@ -663,6 +664,7 @@ static int run_pass(bool adjoint, void* psi, void* lam, int64_t batch, int n, in
   a.gT = threads_log2 - tT;
   a.np = np;
   a.n_gslots = n_gslots;
+  a.flags = 0;
   const int T = 1 << threads_log2;
   const int R = 1 << r;
   int64_t blocks;

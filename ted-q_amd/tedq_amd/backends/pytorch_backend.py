@@ -62,7 +62,7 @@ class _AdjointFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, executor, rows, psi_sink=None):  # pylint: disable=arguments-differ
-        psi = executor.run_forward(rows)
+        psi = executor.run_forward(rows, need_grad=True)
         outs = executor.measure(psi)
         if psi_sink is not None:
             psi_sink.append(psi)

@@ -5,7 +5,11 @@ forward program and the adjoint program for a set of (tile bits k, register
 bits r) shapes with device events and configures the executor with the
 fastest of each.  The shapes trade pass count (larger k: fewer passes) against
 LDS per workgroup (2^k x 8 B forward, 2^k x 16 B adjoint) and register
-pressure (larger r: fewer LDS gather/scatter stages, more VGPRs).
+pressure (larger r: fewer LDS gather/scatter stages, more VGPRs).  A forward
+and an adjoint with the same k share one pass partition, so the forward can
+keep its per-pass states for the adjoint (checkpoints, which spare the adjoint
+a quarter of its traffic); such pairs are timed too and win when their sum is
+lower.
 """
 
 import torch
@@ -69,6 +73,27 @@ def autotune(ex, rows, fwd_shapes=FWD_SHAPES, bwd_shapes=BWD_SHAPES, reps=2, jit
         if log:
             log(f"autotune adjoint {k}/{r}: {bwd_ms[(k, r)]:.2f} ms")
     kb, rb = min(bwd_ms, key=bwd_ms.get)
-    ex.configure(kf, rf, kb, rb)
+    best, best_ms = (kf, rf, kb, rb), fwd_ms[(kf, rf)] + bwd_ms[(kb, rb)]
+    # same-k pairs with checkpoints: the adjoint reads psi from the forward's per-pass states (timed with the
+    # same two state copies as above, so that the sums compare)
+    ckpt_ms = {}
+    if jit:
+        for k in sorted({s[0] for s in fwd_ms} & {s[0] for s in bwd_ms}):
+            rk = min((s[1] for s in fwd_ms if s[0] == k), key=lambda r: fwd_ms[(k, r)])
+            for kb2, rb2 in [s for s in bwd_ms if s[0] == k]:
+                ex.configure(k, rk, kb2, rb2)
+                ex.enable_jit(dev)
+                psi_c = ex.run_forward(rows, need_grad=True)
+                if getattr(psi_c, "tedq_checkpoints", None) is None:
+                    continue
+                lam_c = ex.seed_adjoint(psi_c, ups)
+                ms = _time(lambda: (psi_c.clone(), ex.run_backward(rows, psi_c, lam_c.clone())), reps)
+                ckpt_ms[(k, rk, kb2, rb2)] = fwd_ms[(k, rk)] + ms
+                if log:
+                    log(f"autotune checkpointed {k}/{rk} + {kb2}/{rb2}: adjoint {ms:.2f} ms")
+                if ckpt_ms[(k, rk, kb2, rb2)] < best_ms:
+                    best, best_ms = (k, rk, kb2, rb2), ckpt_ms[(k, rk, kb2, rb2)]
+                del psi_c, lam_c
+    ex.configure(*best)
     ex.jit = keep_jit
-    return {"fwd": fwd_ms, "bwd": bwd_ms, "best": (kf, rf, kb, rb)}
+    return {"fwd": fwd_ms, "bwd": bwd_ms, "checkpointed": ckpt_ms, "best": best}

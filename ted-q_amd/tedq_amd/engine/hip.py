@@ -45,7 +45,7 @@ _SIGNATURES = {
     "tedq_jit_function": ([c_vp, ctypes.c_char_p, c_vp], c_int),
     "tedq_jit_unload": ([c_vp], c_int),
     "tedq_jit_pass": ([c_vp, c_int, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_u64, c_u64, c_vp, c_i64, c_vp, c_i64,
-                       c_int, c_vp, c_vp], c_int),
+                       c_int, c_vp, c_int, c_vp], c_int),
     "tedq_tn_workspace_bytes": ([c_vp, c_i64], c_i64),
     "tedq_tn_contract": ([c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp], c_int),
     "tedq_tn_contract_small_batch": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_int, c_vp], c_int),

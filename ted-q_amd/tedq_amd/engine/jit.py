@@ -225,10 +225,21 @@ def pass_kernel_source(name, words, adjoint, seed_masks=None):
                 L.append(f"    {arr}[sb ^ {offs[m]}] = {v}[{m}];")
         L.append("    __syncthreads();")
         L.append("  }")
-    for m in range(R):
-        L.append(f"  bstore(lds[sw ^ {lds_swizzle(m * T)}], rp, lo * 8u, {int(hi[m]) * 8}u);")
-        if adjoint:
+    if adjoint:
+        # kPassPsiReadOnly: psi is a checkpoint of the forward; the state at this pass's start is the
+        # previous checkpoint, so psi is not written back
+        L.append("  if (!(a.flags & kPassPsiReadOnly)) {")
+        for m in range(R):
+            L.append(f"    bstore(lds[sw ^ {lds_swizzle(m * T)}], rp, lo * 8u, {int(hi[m]) * 8}u);")
+        L.append("  }")
+        for m in range(R):
             L.append(f"  bstore(ldl[sw ^ {lds_swizzle(m * T)}], rl, lo * 8u, {int(hi[m]) * 8}u);")
+    else:
+        # kPassOutOfPlace: the result goes to the second state pointer (a checkpoint), psi keeps its input
+        L.append("  const __amdgpu_buffer_rsrc_t ro = make_rsrc(((a.flags & kPassOutOfPlace) ? lam : psi) + gbase,"
+                 " (uint32_t)(8u << a.n));")
+        for m in range(R):
+            L.append(f"  bstore(lds[sw ^ {lds_swizzle(m * T)}], ro, lo * 8u, {int(hi[m]) * 8}u);")
     L.append("}")
     return "\n".join(L)
 

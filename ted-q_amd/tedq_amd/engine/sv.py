@@ -167,6 +167,16 @@ class _Program:
                 raise RuntimeError("gradient slots of a pass are not contiguous")
             self.pass_gslots.append((min(gs), max(gs) + 1) if gs else (0, 0))
 
+    def checkpoint_fill(self, from_zero):
+        """For the out-of-place forward (forward_passes' ``outs``): whether each output buffer of passes
+        1 .. P-1 must start as zeros.  A pass whose tile holds bits that no earlier pass held reads zeros
+        there which the previous pass, skipping the tiles that hold zeros, never wrote."""
+        seen, fresh = 0, []
+        for mask in self.enc.tile_masks:
+            fresh.append(from_zero and bool(mask & ~seen))
+            seen |= mask
+        return [i + 1 < len(fresh) and fresh[i + 1] for i in range(1, len(fresh))]
+
     def enable_jit(self, device, forward=True, adjoint=True, seed_masks=None):
         """Compile per-pass kernels for this program on ``device`` (hiprtc); returns True if any are in use.
 
@@ -245,33 +255,42 @@ class _Program:
             buf.err_checked = True
         return mats, amats
 
-    def forward_passes(self, psi, mats, probe=None, from_zero=False):
+    def forward_passes(self, psi, mats, probe=None, from_zero=False, outs=None):
         """Apply the passes to psi in place.  ``from_zero``: psi holds |0..0> per row, so each pass skips the
-        tiles that hold zeros (zero_masks)."""
+        tiles that hold zeros (zero_masks).  ``outs`` (generated kernels only): one state buffer per pass
+        after the first; pass i >= 1 reads the previous output and writes outs[i] (out of place), so
+        psi and outs[1:] end up holding the state after every pass (checkpoints)."""
         dev = psi.device
         buf = self.buffers(dev)
         B = psi.shape[0]
         st = hip.stream_handle(dev)
         tl = self.threads_log2()
         fns = self._jit_fns(dev, False)
+        if outs is not None and not fns:
+            raise RuntimeError("checkpointed forward passes need the generated forward kernels")
         for i, (off, mask) in enumerate(zip(self.enc.fwd_offsets, self.enc.tile_masks)):
             zero = self.zero_masks[i] if from_zero else 0
             if probe:
                 probe.begin("sv_pass_fwd")
             if fns:
-                hip.call("tedq_jit_pass", fns[i], 0, hip.ptr(psi), None, B, self.n, self.plan.k, self.plan.r, mask,
-                         zero, hip.ptr(mats), self.plan.mat_stride, None, 0, 0, None, st)
+                src, dst, flags = psi, None, 0
+                if outs is not None and i > 0:
+                    src, dst, flags = outs[i - 1], outs[i], 1
+                hip.call("tedq_jit_pass", fns[i], 0, hip.ptr(src), hip.ptr(dst), B, self.n, self.plan.k, self.plan.r,
+                         mask, zero, hip.ptr(mats), self.plan.mat_stride, None, 0, 0, None, flags, st)
             else:
                 hip.call("tedq_sv_apply_pass", hip.ptr(psi), B, self.n, self.plan.k, self.plan.r, tl,
                          buf.pass_ptr(off), mask, zero, hip.ptr(mats), self.plan.mat_stride, st)
             if probe:
                 probe.end("sv_pass_fwd")
 
-    def adjoint_passes(self, psi, lam, mats, partials, probe=None, from_zero=False, seed=None):
+    def adjoint_passes(self, psi, lam, mats, partials, probe=None, from_zero=False, seed=None, ckpt=None):
         """Reverse sweep over psi and lam (both in place), gradient partial rows into ``partials``.
         ``from_zero``: the forward started from |0..0>, so each pass skips the tiles where psi vanishes;
         the partial rows of skipped tiles are set to zero here.  ``seed`` = (kernel, dy [B, M] float64):
-        the first pass forms lambda from psi with that generated kernel (lam is only written)."""
+        the first pass forms lambda from psi with that generated kernel (lam is only written).
+        ``ckpt``: the forward state after every pass of this same partition (forward_passes' outputs);
+        each pass then reads psi from the checkpoint at its end and leaves it unwritten."""
         dev = psi.device
         buf = self.buffers(dev)
         B = psi.shape[0]
@@ -280,8 +299,8 @@ class _Program:
         rows = self.partial_rows()
         G = self.plan.n_gslots
         fns = self._jit_fns(dev, True)
-        if seed is not None and not fns:
-            raise RuntimeError("a fused adjoint seed needs the generated adjoint kernels")
+        if (seed is not None or ckpt is not None) and not fns:
+            raise RuntimeError("a fused adjoint seed or checkpoints need the generated adjoint kernels")
         P = len(self.plan.passes)
         if from_zero:
             per_tile = rows >> (self.n - self.plan.k)
@@ -295,8 +314,10 @@ class _Program:
                 probe.begin("sv_pass_adj")
             if fns:
                 fn, dy = (seed[0], hip.ptr(seed[1])) if seed is not None and i == 0 else (fns[i], None)
-                hip.call("tedq_jit_pass", fn, 1, hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
-                         mask, zero, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, dy, st)
+                src, flags = (ckpt[P - 1 - i], 2) if ckpt is not None else (psi, 0)
+                hip.call("tedq_jit_pass", fn, 1, hip.ptr(src), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
+                         mask, zero, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, dy, flags,
+                         st)
             else:
                 hip.call("tedq_sv_adjoint_pass", hip.ptr(psi), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
                          tl, buf.pass_ptr(off), mask, zero, hip.ptr(mats), self.plan.mat_stride,
@@ -349,6 +370,9 @@ class SVExecutor:
         self._dev = {}
         # optional timing probe with begin(kind)/end(kind) around every pass launch
         self.probe = None
+        # device bytes the forward may spend on per-pass checkpoints for the backward (None: 40% of the
+        # device's memory; 0: never)
+        self.checkpoint_bytes = None
         # compile per-pass kernels on first use on a device (see enable_jit)
         self.jit = False
         self._jitted = set()
@@ -442,8 +466,28 @@ class SVExecutor:
         return self._dev[key]
 
     # ------------------------------------------------------------------
-    def run_forward(self, params, need_grad=False):  # pylint: disable=unused-argument
-        """Apply the circuit to every parameter row.  Returns psi [B, 2^n] complex64."""
+    def _checkpoints_fit(self, dev, B, from_zero):
+        """Whether a forward that a backward follows keeps the state after every pass: generated kernels
+        for both directions, one pass partition for both (the same tile bits k), every tile written by the
+        last pass, and the extra states within ``checkpoint_bytes`` (None: 40% of the device's memory)."""
+        if self.fwd._jit_fns(dev, False) is None or self.bwd._jit_fns(dev, True) is None:  # pylint: disable=protected-access
+            return False
+        P = len(self.fwd.plan.passes)
+        if P < 2 or self.fwd.enc.tile_masks != self.bwd.enc.tile_masks:
+            return False
+        if from_zero and self.fwd.zero_masks[-1]:
+            return False
+        budget = self.checkpoint_bytes
+        if budget is None:
+            budget = int(0.4 * torch.cuda.get_device_properties(dev).total_memory)
+        return (P - 1) * B * (8 << self.n) <= budget
+
+    def run_forward(self, params, need_grad=False):
+        """Apply the circuit to every parameter row.  Returns psi [B, 2^n] complex64.
+
+        With ``need_grad`` (a backward follows) and when they fit (_checkpoints_fit), the passes write
+        out of place and the states after every pass stay attached to the returned psi as checkpoints:
+        the adjoint sweep then reads psi from them instead of writing it back (a quarter less traffic)."""
         dev = params.device
         B = params.shape[0]
         self._ensure_jit(dev)
@@ -455,8 +499,18 @@ class SVExecutor:
             hip.call("tedq_sv_init_zero", hip.ptr(psi), B, self.n, st)
         else:
             hip.call("tedq_sv_init_copy", hip.ptr(psi), hip.ptr(init), B, self.n, st)
-        self.fwd.forward_passes(psi, mats, self.probe, from_zero=init is None)
-        return psi
+        outs = None
+        if need_grad and self._checkpoints_fit(dev, B, init is None):
+            fill = self.fwd.checkpoint_fill(init is None)
+            outs = [psi] + [torch.zeros_like(psi) if z else torch.empty_like(psi) for z in fill]
+        self.fwd.forward_passes(psi, mats, self.probe, from_zero=init is None, outs=outs)
+        if outs is None:
+            return psi
+        final = outs[-1]
+        # the adjoint program they serve and the states before the last pass's output (which is psi itself:
+        # holding it here would make a reference cycle that keeps every state alive until a collection)
+        final.tedq_checkpoints = (self.bwd, outs[:-1])
+        return final
 
     def measure(self, psi):
         """List of per-measurement outputs, each [B, *shape] (expvals fp64, probs fp32, state c64)."""
@@ -657,7 +711,10 @@ class SVExecutor:
         G = prog.plan.n_gslots
         rows = prog.partial_rows()
         partials = torch.empty((B, max(G, 1), rows, 8), dtype=torch.float32, device=dev)
-        prog.adjoint_passes(psi, lam, mats, partials, self.probe, from_zero=self.init_state is None, seed=seed)
+        ck = getattr(psi, "tedq_checkpoints", None)
+        ckpt = list(ck[1]) + [psi] if ck is not None and ck[0] is prog else None
+        prog.adjoint_passes(psi, lam, mats, partials, self.probe, from_zero=self.init_state is None, seed=seed,
+                            ckpt=ckpt)
         C = torch.empty((B, max(G, 1), 8), dtype=torch.float64, device=dev)
         buf = prog.buffers(dev)
         hip.call("tedq_sv_grad_reduce", hip.ptr(partials), B, G, rows, hip.ptr(amats), hip.ptr(buf.pgslot),
@@ -671,7 +728,7 @@ class SVExecutor:
         ``grads_out`` holds one upstream gradient per measurement (None = 0).
         Returns (list of measurement outputs, grad [B, P]).
         """
-        psi = self.run_forward(rows)
+        psi = self.run_forward(rows, need_grad=True)
         outs = self.measure(psi)
         grad = self.adjoint(rows, psi, grads_out, grad_dtype=rows.dtype)
         return outs, grad

@@ -221,6 +221,27 @@ def run_pass(words, n, states, mats, acc, zero=0):
             s[base + local_phys] = tile
 
 
+def run_pass_out(words, n, src, dst, mats, zero=0):
+    """One forward pass read from ``src`` and written to ``dst`` (out of place), tiles meeting ``zero``
+    skipped (neither read nor written)."""
+    k = int(words[0])
+    tile_bits = [int(b) for b in words[P.HDR_TILE_BITS:P.HDR_TILE_BITS + k]]
+    rest = [b for b in range(n) if b not in tile_bits]
+    local_phys = np.array([_pdep(j, tile_bits) for j in range(1 << k)], dtype=np.int64)
+    work = np.zeros_like(src)
+    for t in range(1 << (n - k)):
+        base = _pdep(t, rest)
+        if base & zero:
+            continue
+        work[base + local_phys] = src[base + local_phys]
+    run_pass(words, n, [work], mats, {}, zero)
+    for t in range(1 << (n - k)):
+        base = _pdep(t, rest)
+        if base & zero:
+            continue
+        dst[base + local_phys] = work[base + local_phys]
+
+
 def forward(enc, plan, row, init=None, zero_masks=None):
     n = plan.n
     mats, amats = build_tables(enc, plan, row)

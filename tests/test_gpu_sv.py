@@ -761,3 +761,36 @@ def test_adjoint_seed_formed_in_the_first_generated_pass():
     e_g = relerr(g_fused[2].cpu().numpy(), t64.grad.numpy(), GFLOOR)
     print(f"fused adjoint seed: gradient relerr {e_g:.2e}", flush=True)
     assert e_g < TOL
+
+
+def test_adjoint_reads_forward_checkpoints():
+    """Forward and adjoint with one pass partition (same tile bits): the forward keeps the state after every
+    pass and the adjoint reads psi from those checkpoints instead of writing it back.  Gradients equal the
+    in-place sweep's (checkpoint_bytes = 0) to 1e-6 and the oracle's (1e-5 relative), with and without the
+    fused seed, and the returned state equals the in-place forward's."""
+    from tedq_amd import workloads as W
+    n, L = 18, 6
+    c = W.hea(n, L, jit=True, tile_bits=11, reg_bits=4, tile_bits_bwd=11, reg_bits_bwd=3)
+    ex = c.executor()
+    rng = np.random.default_rng(101)
+    B = 3
+    vals = rng.uniform(0, 2 * np.pi, size=(B, 2 * n * L))
+    rows = torch.tensor(vals, dtype=torch.float32, device=DEV)
+    ups = [torch.tensor(rng.normal(size=B), dtype=torch.float32, device=DEV) for _ in range(n)]
+    psi_c = ex.run_forward(rows, need_grad=True)
+    assert getattr(psi_c, "tedq_checkpoints", None) is not None and len(ex.fwd.plan.passes) > 2
+    ex.checkpoint_bytes = 0
+    psi_i = ex.run_forward(rows, need_grad=True)
+    assert getattr(psi_i, "tedq_checkpoints", None) is None
+    assert torch.equal(psi_c, psi_i)
+    g_inplace = ex.adjoint(rows, psi_i, ups)
+    g_ckpt = ex.adjoint(rows, psi_c, ups)                       # fused seed + checkpoints
+    g_ckpt2 = ex.run_backward(rows, psi_c, ex.seed_adjoint(psi_c, ups))   # separate seed + checkpoints
+    assert np.max(np.abs(g_ckpt.cpu().numpy() - g_inplace.cpu().numpy())) < 1e-6
+    assert np.max(np.abs(g_ckpt2.cpu().numpy() - g_inplace.cpu().numpy())) < 1e-6
+    t64 = torch.tensor(vals[1], dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, O.hea_gates(n, t64.reshape(L, n, 2))[0], [("expval", ("PauliZ", [q])) for q in range(n)])
+    (ref * torch.tensor([float(u[1]) for u in ups], dtype=torch.float64)).sum().backward()
+    e_g = relerr(g_ckpt[1].cpu().numpy(), t64.grad.numpy(), GFLOOR)
+    print(f"checkpointed adjoint: gradient relerr {e_g:.2e}", flush=True)
+    assert e_g < TOL

@@ -368,3 +368,47 @@ def test_passes_skip_tiles_that_hold_zeros(seed, n, k, r):
             used = {prog.plan.slots[op[4]].gslot for st in p.stages for op in st.ops
                     if op[4] >= 0 and prog.plan.slots[op[4]].gslot >= 0}
             assert used == set(range(g0, g1))
+
+
+@pytest.mark.parametrize("seed,n,k,r", [(40, 8, 4, 2), (41, 9, 5, 3), (42, 10, 5, 2)])
+def test_checkpointed_forward_buffers(seed, n, k, r):
+    """Out-of-place forward with tile skipping (the checkpoints the adjoint reads): with the output buffers
+    the executor zero-fills (checkpoint_fill) and the rest uninitialised (NaN here), every checkpoint
+    equals the in-place state after that pass on the tiles the adjoint reads (its live tiles), and the
+    last one everywhere."""
+    from tedq_amd.engine.sv import _Program
+    rng = np.random.default_rng(seed)
+    spec = random_spec(n, 40, rng)
+    npar = sum(s[2] for s in spec)
+    values = np.asarray([float(x) for x in rng.uniform(-math.pi, math.pi, npar)])
+    circ = trace(n, spec, values, lambda: qai.expval(qai.PauliZ(qubits=[0])))
+    nq, units, Pn, pool = lower_circuit(circ.compilecircuit(backend="pytorch"))
+    prog = _Program(nq, units, Pn, pool, k, r, specialize=True)
+    P = len(prog.plan.passes)
+    fill = prog.checkpoint_fill(True)
+    assert len(fill) == P - 1
+    mats, _ = EMU.build_tables(prog.enc, prog.plan, values)
+    # in place, every tile: the states after each pass
+    ref, states = np.zeros(2 ** n, dtype=np.complex128), []
+    ref[0] = 1
+    for off in prog.enc.fwd_offsets:
+        EMU.run_pass(prog.enc.prog[off:], n, [ref], mats, {})
+        states.append(ref.copy())
+    # out of place with skipping
+    psi = np.zeros(2 ** n, dtype=np.complex128)
+    psi[0] = 1
+    outs = [psi] + [np.zeros(2 ** n, dtype=np.complex128) if z else np.full(2 ** n, np.nan, dtype=np.complex128)
+                    for z in fill]
+    for i, off in enumerate(prog.enc.fwd_offsets):
+        if i == 0:
+            EMU.run_pass(prog.enc.prog[off:], n, [psi], mats, {}, prog.zero_masks[0])
+        else:
+            EMU.run_pass_out(prog.enc.prog[off:], n, outs[i - 1], outs[i], mats, prog.zero_masks[i])
+    for i in range(P):
+        words = prog.enc.prog[prog.enc.fwd_offsets[i]:]
+        kk = int(words[0])
+        tile = [int(b) for b in words[4:4 + kk]]
+        idx = np.arange(2 ** n)
+        live = (idx & prog.zero_masks[i]) == 0
+        assert np.allclose(outs[i][live], states[i][live], atol=1e-12), i
+    assert prog.zero_masks[-1] != 0 or np.allclose(outs[-1], states[-1], atol=1e-12)
