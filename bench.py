@@ -368,10 +368,12 @@ def measured_traffic(kernel, n, layers, batch, ex):
         return None
     with open(path) as f:
         recs = json.load(f)
+    ckpt = ex._checkpoints_fit(torch.device("cuda", 0), batch, True)  # pylint: disable=protected-access
     for rec in recs if isinstance(recs, list) else [recs]:
         w = rec["workload"]
         if (rec["kernel"] == kernel and w["qubits"] == n and w["layers"] == layers and w["batch"] == batch
-                and w["tile_bits_bwd"] == ex.bwd.plan.k and w["reg_bits_bwd"] == ex.bwd.plan.r):
+                and w["tile_bits_bwd"] == ex.bwd.plan.k and w["reg_bits_bwd"] == ex.bwd.plan.r
+                and w.get("forward_checkpoints", False) == ckpt):
             return rec["traffic_bytes_per_launch"]
     return None
 
