@@ -347,8 +347,8 @@ def serving_leg(n_req, warm, ragged=True, common_rate=None):
            "capacity_evals_per_s": cap["evals_per_s"], "mean_batch": mean_batch,
            "closed_loop_mean_batch": closed_batch, "ragged_batches": eng.ragged_batches,
            "unbatched_evals_per_s": un["evals_per_s"], "graphs_captured": len(eng._graphs),
-           "batching": "ragged once 8 requests of a tile shape wait: requests of different structures "
-                       "sharing a tile shape run in one launch per pass index; below that, and for "
+           "batching": "ragged once a full batch (64) of a tile shape waits: requests of different "
+                       "structures sharing a tile shape run in one launch per pass index; below that, and for "
                        "identical structures, the oldest request's structure runs at once on captured graphs",
            "stream": ("Poisson at 0.7x the same-structure engine's closed-loop capacity (high_load: at 0.7x "
                       "this engine's own)" if high else "Poisson at 0.7x closed-loop capacity")

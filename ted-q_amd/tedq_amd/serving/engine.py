@@ -9,12 +9,13 @@ the circuit's measurements and the full parameter gradient.  The loop
   identical circuits from whatever is waiting when a slot frees up, oldest
   request first;
 * with ``ragged=True`` queues by tile shape instead: once ``ragged_min``
-  requests of a tile shape wait, a batch may mix circuit structures (depths
-  and qubit counts), and every pass index of all of them runs as one ragged
-  launch (tedq_amd/engine/ragged.py); with fewer waiting (light load), or when
-  the oldest requests share one structure, the oldest request's structure is
-  served at once on the same-structure path (captured graphs), which costs the
-  host far less per batch;
+  requests of a tile shape wait (a backlog: by default a full batch), a batch
+  may mix circuit structures (depths and qubit counts), and every pass index
+  of all of them runs as one ragged launch (tedq_amd/engine/ragged.py); with
+  fewer waiting, or when the oldest requests share one structure, the oldest
+  request's structure is served at once on the same-structure path (captured
+  graphs), which costs the host far less per batch, so latency under moderate
+  load stays that of the same-structure loop;
 * admits a batch only while the bytes of all in-flight batches stay within an
   HBM budget (requests wait otherwise);
 * runs batches on several HIP streams so their kernels overlap on the device,
@@ -114,11 +115,14 @@ class ServingEngine:
     """
 
     def __init__(self, factory, device=None, hbm_budget=32 << 30, max_batch=64, n_streams=4, max_inflight=8,
-                 graph_max_qubits=18, ragged=False, ragged_min=8):
+                 graph_max_qubits=22, ragged=False, ragged_min=None, ragged_max=64):
         self.factory = factory
-        # queue by tile shape and mix structures in one launch sequence (see module docstring)
+        # queue by tile shape and mix structures in one launch sequence (see module docstring) once
+        # ragged_min requests of a shape wait (None: a full batch, max_batch); a ragged batch holds at most
+        # ragged_max requests (its host-side cost grows with the structures in it)
         self.ragged = ragged
-        self.ragged_min = ragged_min
+        self.ragged_min = max_batch if ragged_min is None else ragged_min
+        self.ragged_max = ragged_max
         self.ragged_batches = 0
         self.graph_max_qubits = graph_max_qubits
         self._graphs = {}
@@ -278,7 +282,7 @@ class ServingEngine:
         from ..engine.ragged import ragged_value_and_grad  # pylint: disable=import-outside-toplevel
         q = self._queues[qkey]
         reqs, need = [], 0
-        while q and len(reqs) < self.max_batch:
+        while q and len(reqs) < min(self.max_batch, self.ragged_max):
             more = self.executor(q[0].key).memory_bytes(1)
             if reqs and self._bytes_in_use + self.graph_bytes + need + more > self.hbm_budget:
                 break
