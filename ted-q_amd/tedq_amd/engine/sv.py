@@ -472,15 +472,23 @@ class SVExecutor:
         last pass, and the extra states within ``checkpoint_bytes`` (None: 40% of the device's memory)."""
         if self.fwd._jit_fns(dev, False) is None or self.bwd._jit_fns(dev, True) is None:  # pylint: disable=protected-access
             return False
-        P = len(self.fwd.plan.passes)
-        if P < 2 or self.fwd.enc.tile_masks != self.bwd.enc.tile_masks:
+        if not self.checkpoints_compatible():
             return False
         if from_zero and self.fwd.zero_masks[-1]:
             return False
         budget = self.checkpoint_bytes
         if budget is None:
             budget = int(0.4 * torch.cuda.get_device_properties(dev).total_memory)
-        return (P - 1) * B * (8 << self.n) <= budget
+        return (len(self.fwd.plan.passes) - 1) * B * (8 << self.n) <= budget
+
+    def checkpoints_compatible(self):
+        """Whether the forward's state after each pass is the state the adjoint walks back to: at least two
+        passes, one pass partition, and one phase convention.  A specialised plan applies diagonal gates
+        without their global phase; the in-place sweep does not care (psi and lambda carry the same
+        phases), but an exact forward's checkpoints would meet a lambda whose phases differ."""
+        f, b = self.fwd, self.bwd
+        return (len(f.plan.passes) >= 2 and f.enc.tile_masks == b.enc.tile_masks
+                and f.plan.specialize == b.plan.specialize)
 
     def run_forward(self, params, need_grad=False):
         """Apply the circuit to every parameter row.  Returns psi [B, 2^n] complex64.

@@ -255,11 +255,15 @@ def forward(enc, plan, row, init=None, zero_masks=None):
     return psi, mats, amats
 
 
-def adjoint_grad(enc, plan, psi, lam, mats, amats, n_params, zero_masks=None):
+def adjoint_grad(enc, plan, psi, lam, mats, amats, n_params, zero_masks=None, ckpt=None):
+    """Reverse sweep; ``ckpt`` (the forward's state after every pass): each pass reads psi from the
+    checkpoint at its end instead of the state the previous adjoint pass left."""
     acc = {}
     psi, lam = psi.copy(), lam.copy()
     P = len(enc.adj_offsets)
     for i, off in enumerate(enc.adj_offsets):
+        if ckpt is not None:
+            psi = ckpt[P - 1 - i].copy()
         run_pass(enc.prog[off:], plan.n, [psi, lam], mats, acc, zero_masks[P - 1 - i] if zero_masks else 0)
     grad = np.zeros(n_params)
     for p in range(n_params):

@@ -794,3 +794,59 @@ def test_adjoint_reads_forward_checkpoints():
     e_g = relerr(g_ckpt[1].cpu().numpy(), t64.grad.numpy(), GFLOOR)
     print(f"checkpointed adjoint: gradient relerr {e_g:.2e}", flush=True)
     assert e_g < TOL
+
+
+@pytest.mark.parametrize("kind", ["obs", "probs"])
+def test_checkpoints_with_initial_state_and_nonz_outputs(kind):
+    """Checkpointed forward and adjoint (generated kernels, one pass partition) started from a user state
+    (IintStateVector, prepared_state.py:29-57; no tile holds zeros, so every checkpoint is written) with
+    outputs the adjoint seeds through the basis change or the probabilities (pytorch_backend.py:393-498):
+    values and gradients against the oracle, gradients against the in-place sweep (checkpoint_bytes 0).
+    Probabilities keep the exact forward, whose states differ by phases from the generated adjoint's
+    (diagonal gates without their global phase): that pair runs without checkpoints."""
+    n = 13
+    rng = np.random.default_rng(131 if kind == "obs" else 137)
+    init = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    init /= np.linalg.norm(init)
+    spec = _random_spec(n, 90, rng)
+    npar = sum(s[2] for s in spec)
+    if kind == "obs":
+        meas = lambda: [qai.expval(qai.PauliZ(qubits=[0])), qai.expval(qai.PauliX(qubits=[3])),  # noqa: E731
+                        qai.expval(qai.PauliY(qubits=[11]))]
+        oracle_meas = [("expval", ("PauliZ", [0])), ("expval", ("PauliX", [3])), ("expval", ("PauliY", [11]))]
+    else:
+        meas = lambda: qai.probs(qubits=[7, 2])  # noqa: E731
+        oracle_meas = [("probs", [7, 2])]
+
+    def cd(*p):
+        qai.IintStateVector(init.reshape([2] * n))
+        it = iter(p)
+        for name, qubits, k in spec:
+            getattr(qai, name)(*[next(it) for _ in range(k)], qubits=qubits)
+        return meas()
+    c = qai.Circuit(cd, n, *([0.1] * npar)).compilecircuit(backend="pytorch", jit=True, tile_bits=10, reg_bits=4,
+                                                           tile_bits_bwd=10, reg_bits_bwd=3)
+    vals = rng.uniform(-math.pi, math.pi, npar)
+    print(f"case kind={kind} n={n} gates={len(spec)} params={npar}", flush=True)
+    grads = []
+    for budget in (None, 0):
+        ex = c.executor()
+        ex.checkpoint_bytes = budget
+        probe = ex.run_forward(torch.tensor(vals[None], dtype=torch.float32, device=DEV), need_grad=True)
+        assert (getattr(probe, "tedq_checkpoints", None) is not None) == (budget is None and kind == "obs")
+        assert len(ex.fwd.plan.passes) > 2
+        del probe
+        t = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+        out = c(t)
+        dy = torch.tensor(np.random.default_rng(7).normal(size=tuple(out.shape)), dtype=torch.float32, device=DEV)
+        (out * dy).sum().backward()
+        grads.append(t.grad.cpu().numpy())
+    t64 = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    ref = O.execute(n, _oracle_gates(spec, list(t64)), oracle_meas, init=init)
+    (ref * dy.double().cpu().reshape(ref.shape)).sum().backward()
+    e_v = relerr(out.detach().cpu().numpy().reshape(-1), ref.detach().numpy().reshape(-1))
+    e_g = relerr(grads[0], t64.grad.numpy(), GFLOOR)
+    print(f"checkpoints from a user state ({kind}): values relerr {e_v:.2e}, gradients relerr {e_g:.2e}, "
+          f"vs in-place {np.max(np.abs(grads[0] - grads[1])):.2e}", flush=True)
+    assert e_v < TOL and e_g < TOL
+    assert np.max(np.abs(grads[0] - grads[1])) < 1e-5

@@ -412,3 +412,54 @@ def test_checkpointed_forward_buffers(seed, n, k, r):
         live = (idx & prog.zero_masks[i]) == 0
         assert np.allclose(outs[i][live], states[i][live], atol=1e-12), i
     assert prog.zero_masks[-1] != 0 or np.allclose(outs[-1], states[-1], atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_checkpoints_need_one_phase_convention(seed):
+    """The specialised adjoint applies diagonal gates without their global phase.  Walking back in place
+    it is exact (psi and lambda carry the same phases); reading psi from checkpoints it is exact only when
+    the forward that wrote them used the same convention.  The exact forward's checkpoints (kept for
+    probabilities or a returned state) give a wrong gradient, so the executor keeps none for that pair."""
+    from tedq_amd.engine.sv import _Program
+    n, k, r = 7, 5, 3
+    rng = np.random.default_rng(seed)
+    spec = [("RZ", [q], 1) for q in range(n)] + random_spec(n, 40, rng) + [("RY", [q], 1) for q in range(n)]
+    npar = sum(s[2] for s in spec)
+    values = np.asarray([float(x) for x in rng.uniform(-math.pi, math.pi, npar)])
+    circ = trace(n, spec, values, lambda: qai.expval(qai.PauliZ(qubits=[0])))
+    nq, units, Pn, pool = lower_circuit(circ.compilecircuit(backend="pytorch"))
+    exact = _Program(nq, units, Pn, pool, k, r)
+    spec_p = _Program(nq, units, Pn, pool, k, r, specialize=True)
+    assert exact.enc.tile_masks == spec_p.enc.tile_masks and len(spec_p.plan.passes) >= 2
+
+    def states(prog):
+        mats, _ = EMU.build_tables(prog.enc, prog.plan, values)
+        psi, out = np.zeros(2 ** n, dtype=np.complex128), []
+        psi[0] = 1
+        for off in prog.enc.fwd_offsets:
+            EMU.run_pass(prog.enc.prog[off:], n, [psi], mats, {})
+            out.append(psi.copy())
+        return out
+    ck_exact, ck_spec = states(exact), states(spec_p)
+    mats, amats = EMU.build_tables(spec_p.enc, spec_p.plan, values)
+    z0 = 1 - 2 * ((np.arange(2 ** n) >> (n - 1)) & 1)
+    t = torch.tensor(values, dtype=torch.float64, requires_grad=True)
+    O.execute(n, oracle_gates(spec, list(t)), [("expval", ("PauliZ", [0]))]).sum().backward()
+    ref = t.grad.numpy()
+    for ck in (ck_exact, ck_spec):
+        g_inplace, _ = EMU.adjoint_grad(spec_p.enc, spec_p.plan, ck[-1], z0 * ck[-1], mats, amats, npar)
+        assert np.allclose(g_inplace, ref, atol=1e-9)
+    g_spec, _ = EMU.adjoint_grad(spec_p.enc, spec_p.plan, ck_spec[-1], z0 * ck_spec[-1], mats, amats, npar,
+                                 ckpt=ck_spec)
+    g_mixed, _ = EMU.adjoint_grad(spec_p.enc, spec_p.plan, ck_exact[-1], z0 * ck_exact[-1], mats, amats, npar,
+                                  ckpt=ck_exact)
+    assert np.allclose(g_spec, ref, atol=1e-9)
+    assert np.max(np.abs(g_mixed - ref)) > 1e-3
+    # the executor pairs them only under one convention: an exact forward for probabilities, a specialised
+    # one (as generated kernels run it) when every output is an expectation
+    for meas, ok in ((lambda: qai.probs(qubits=[1, 2]), False), (lambda: qai.expval(qai.PauliX(qubits=[3])), True)):
+        ex = trace(n, spec, values, meas).compilecircuit(backend="pytorch", tile_bits=k, reg_bits=r,
+                                                          fold=False).executor()
+        ex.fwd = ex._program(k, r, specialize=ex.phase_free)   # pylint: disable=protected-access
+        ex.bwd = ex._program(k, r, specialize=True)            # pylint: disable=protected-access
+        assert ex.checkpoints_compatible() == ok
