@@ -353,8 +353,10 @@ class SVExecutor:
                 s.zmask = self.zmasks[s.zindex]
         n = self.n
         self._units = units
-        # expectation values only: the forward may drop global phases (see enable_jit)
-        self.phase_free = bool(self.specs) and all(s.kind in ("zexp", "fexp") for s in self.specs)
+        # expectation values and probabilities only (no state returned, none taken after a measurement):
+        # the forward may drop global phases (see enable_jit)
+        self.phase_free = (bool(self.specs) and all(s.kind in ("zexp", "fexp", "probs") for s in self.specs)
+                           and not any(getattr(m, "after_state", False) for m in measurements))
         # every forward measurement also reduces <psi|psi> in fp64 (the Z-string of mask 0, compensated
         # sums), appended after the circuit's own masks; the last batch's values stay in last_norm
         self._zall = list(self.zmasks) + [0]
@@ -384,8 +386,9 @@ class SVExecutor:
         The adjoint switches to its specialised plan (unfused rotations with single-value gradients,
         real 2x2 ops, diagonal gates without their global phase) when generated kernels can run it:
         that plan has about twice the ops, which only generated code executes without per-op overhead.
-        The forward does the same when every measurement is an expectation value (which no global
-        phase changes; a returned state or probabilities keep the exact forward).  Returns
+        The forward does the same when every measurement is an expectation value or a probability
+        (which no global phase changes; a returned state or states after a measurement keep the exact
+        forward).  Returns
         (forward_jitted, adjoint_jitted).  Compiled programs are kept per (k, r) shape.
         """
         from . import jit  # pylint: disable=import-outside-toplevel

@@ -419,7 +419,7 @@ def test_checkpoints_need_one_phase_convention(seed):
     """The specialised adjoint applies diagonal gates without their global phase.  Walking back in place
     it is exact (psi and lambda carry the same phases); reading psi from checkpoints it is exact only when
     the forward that wrote them used the same convention.  The exact forward's checkpoints (kept for
-    probabilities or a returned state) give a wrong gradient, so the executor keeps none for that pair."""
+    a returned state) give a wrong gradient, so the executor keeps none for that pair."""
     from tedq_amd.engine.sv import _Program
     n, k, r = 7, 5, 3
     rng = np.random.default_rng(seed)
@@ -455,9 +455,11 @@ def test_checkpoints_need_one_phase_convention(seed):
                                   ckpt=ck_exact)
     assert np.allclose(g_spec, ref, atol=1e-9)
     assert np.max(np.abs(g_mixed - ref)) > 1e-3
-    # the executor pairs them only under one convention: an exact forward for probabilities, a specialised
-    # one (as generated kernels run it) when every output is an expectation
-    for meas, ok in ((lambda: qai.probs(qubits=[1, 2]), False), (lambda: qai.expval(qai.PauliX(qubits=[3])), True)):
+    # the executor pairs them only under one convention: an exact forward when the state is returned or
+    # taken after a measurement, a specialised one (as generated kernels run it) for expectations and
+    # probabilities
+    for meas, ok in ((lambda: qai.probs(qubits=[1, 2]), True), (lambda: qai.expval(qai.PauliX(qubits=[3])), True),
+                     (lambda: qai.probs(qubits=[1, 2], after_state=True), False), (lambda: qai.state(), False)):
         ex = trace(n, spec, values, meas).compilecircuit(backend="pytorch", tile_bits=k, reg_bits=r,
                                                           fold=False).executor()
         ex.fwd = ex._program(k, r, specialize=ex.phase_free)   # pylint: disable=protected-access
