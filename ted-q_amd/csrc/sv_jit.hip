@@ -128,7 +128,7 @@ extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_
   }
   const unsigned T = 1u << (k - r);
   const size_t lds = (size_t)(1u << k) * sizeof(float2) * (adjoint ? 2 : 1);
-  if (lds > 65536 || a.total_tiles > 0x7fffffffll) {
+  if (lds > 160 * 1024 || a.total_tiles > 0x7fffffffll) {
     set_error("tedq_jit_pass: tile of %zu LDS bytes or %lld tiles is out of range", lds, (long long)a.total_tiles);
     return kBadArgument;
   }

@@ -15,7 +15,7 @@ lower.
 import torch
 
 FWD_SHAPES = ((12, 4), (13, 4), (13, 5), (12, 5), (11, 4), (12, 3))
-BWD_SHAPES = ((12, 4), (12, 3), (11, 4), (11, 3), (10, 3), (10, 2))
+BWD_SHAPES = ((12, 4), (12, 3), (13, 4), (11, 4), (11, 3), (10, 3), (10, 2))
 
 
 def valid_shape(k, r, adjoint):

@@ -31,7 +31,7 @@ _HEADERS = ("tedq_common.h", "sv_ops.h")
 
 def supported(n, k, r, adjoint):
     lds = (1 << k) * 8 * (2 if adjoint else 1)
-    return k < n <= 28 and lds <= 65536 and k - r <= 9 and 1 <= r <= 5
+    return k < n <= 28 and lds <= 160 * 1024 and k - r <= 9 and 1 <= r <= 5
 
 
 def _bits(mask):

@@ -408,6 +408,9 @@ def _hea20_shapes():
     shapes = [(12, 4, 12, 3)]
     shapes += [(k, r, 12, 3) for k, r in FWD_SHAPES if (k, r) != (12, 4)]
     shapes += [(12, 4, k, r) for k, r in BWD_SHAPES if (k, r) != (12, 3)]
+    # pairs with one tile size (the adjoint then reads the forward's checkpoints)
+    shapes += [(k, r, kb, rb) for k, r in FWD_SHAPES for kb, rb in BWD_SHAPES
+               if k == kb and (k, r, kb, rb) not in shapes]
     return shapes
 
 
