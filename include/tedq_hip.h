@@ -168,13 +168,17 @@ int tedq_jit_function(void* module, const char* name, void** fn_out);
 int tedq_jit_unload(void* module);
 /* Launch one generated pass (forward when adjoint == 0) over every circuit;
  * arguments as tedq_sv_apply_pass / tedq_sv_adjoint_pass (k < n <= 28, at most
- * 64 KiB of LDS per workgroup).  seed: dev double [batch][M], read only by a
+ * 160 KiB of LDS per workgroup).  seed: dev double [batch][M], read only by a
  * generated first adjoint pass that forms lambda from psi (the Z-string
  * adjoint seed, tedq_sv_lambda_z, fused into the pass); null otherwise.
  * flags: 1 (forward) reads psi and writes the result to lam, a separate state
  * buffer (the forward keeps every pass's output as a checkpoint); 2 (adjoint)
  * psi is the checkpoint of the forward state at the end of this pass and is
- * only read (the state at its start is the previous checkpoint).  0: in place. */
+ * only read (the state at its start is the previous checkpoint); 4 (forward,
+ * with the zero mask of a sweep from |0..0>) takes the amplitudes at tile bits
+ * no earlier pass held as zero instead of reading them, and the first pass
+ * builds the |0..0> tile without reading psi, so psi needs no initialisation
+ * when the last pass writes every tile.  0: in place. */
 int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits,
                   int reg_bits, uint64_t tile_mask, uint64_t zero_mask, const void* mats, int64_t mat_stride,
                   void* partials, int64_t partial_rows, int n_gslots, const double* seed, int flags, void* stream);

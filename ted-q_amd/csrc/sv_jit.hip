@@ -121,8 +121,9 @@ extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_
   a.np = partial_rows;
   a.n_gslots = n_gslots;
   a.flags = flags;
-  if ((flags & ~(kPassOutOfPlace | kPassPsiReadOnly)) != 0 || (!adjoint && (flags & kPassOutOfPlace) && lam == nullptr) ||
-      (adjoint && (flags & kPassOutOfPlace)) || (!adjoint && (flags & kPassPsiReadOnly))) {
+  if ((flags & ~(kPassOutOfPlace | kPassPsiReadOnly | kPassFromZero)) != 0 ||
+      (!adjoint && (flags & kPassOutOfPlace) && lam == nullptr) || (adjoint && (flags & (kPassOutOfPlace | kPassFromZero))) ||
+      (!adjoint && (flags & kPassPsiReadOnly))) {
     set_error("tedq_jit_pass: flags %d do not fit a %s pass", flags, adjoint ? "adjoint" : "forward");
     return kBadArgument;
   }

@@ -45,7 +45,7 @@ struct PassArgs {
   int gT;                   // log2(tiles per workgroup)
   int64_t np;               // adjoint: partials are [batch][n_gslots][np][8] (partial rows per slot per circuit)
   int n_gslots;
-  int flags;                // generated passes: kPassOutOfPlace, kPassPsiReadOnly
+  int flags;                // generated passes: kPassOutOfPlace, kPassPsiReadOnly, kPassFromZero
 };
 
 // Generated forward pass: read psi, write the result to the second state pointer (a checkpoint).
@@ -53,6 +53,10 @@ constexpr int kPassOutOfPlace = 1;
 // Generated adjoint pass: psi holds the forward state at the end of this pass (a checkpoint) and is only
 // read; the state at its start is the previous checkpoint, so psi is not written back.
 constexpr int kPassPsiReadOnly = 2;
+// Generated forward pass of a sweep from |0..0> (with the zero mask of tile skipping): tile bits that no
+// earlier pass held are 0 wherever the state is nonzero, so those amplitudes are taken as zero instead of
+// loaded (the first pass, whose tile bits are all new, builds the |0..0> tile without reading psi).
+constexpr int kPassFromZero = 4;
 
 // ---------------------------------------------------------------------------
 // register-level gate kernels.  B is the target register bit and CM a control

@@ -160,12 +160,15 @@ def _seed_lines(hi, R, masks):
     return out
 
 
-def pass_kernel_source(name, words, adjoint, seed_masks=None):
+def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0):
     """HIP source of one generated pass kernel from its program words (program.py layout).
 
     ``seed_masks`` (adjoint only): instead of loading lambda, form it from the loaded psi as
     lambda(x) = sum_j seed[circuit][j] (-1)^popc(x & mask_j) psi(x) -- the adjoint seed of Z-string
-    expectation values -- for the first pass of the sweep."""
+    expectation values -- for the first pass of the sweep.
+    ``fresh`` (forward only): the tile bits no earlier pass held.  Under kPassFromZero (a sweep from
+    |0..0>) the state is zero wherever one of them is 1, so those amplitudes are set instead of loaded;
+    when every tile bit is fresh (the first pass) the tile is |0..0> itself and nothing is loaded."""
     words = np.asarray(words, dtype=np.int32)
     k, r, nst, tT = (int(x) for x in words[:4])
     R, T = 1 << r, 1 << tT
@@ -200,10 +203,25 @@ def pass_kernel_source(name, words, adjoint, seed_masks=None):
             L.append(f"  {{ const float2 v = bload(rp, lo * 8u, {int(hi[m]) * 8}u); lds[sw ^ {x}] = v;"
                      f" ldl[sw ^ {x}] = make_float2(d{m} * v.x, d{m} * v.y); }}")
     L.append("  (void)seed;")
+    tile_mask = sum(1 << b for b in tile_bits)
+    if not adjoint and fresh:
+        L.append("  if (a.flags & kPassFromZero) {")
+        for m in range(R):
+            x = lds_swizzle(m * T)
+            if fresh == tile_mask:
+                L.append(f"    lds[sw ^ {x}] = make_float2({'tid == 0 ? 1.f : 0.f' if m == 0 else '0.f'}, 0.f);")
+            elif int(hi[m]) & fresh:
+                L.append(f"    lds[sw ^ {x}] = make_float2(0.f, 0.f);")
+            else:
+                L.append(f"    {{ float2 v = make_float2(0.f, 0.f); if ((lo & {fresh}u) == 0u) v = bload(rp, lo * 8u, "
+                         f"{int(hi[m]) * 8}u); lds[sw ^ {x}] = v; }}")
+        L.append("  } else {")
     for m in range(R if seed_masks is None or not adjoint else 0):
         L.append(f"  lds[sw ^ {lds_swizzle(m * T)}] = bload(rp, lo * 8u, {int(hi[m]) * 8}u);")
         if adjoint:
             L.append(f"  ldl[sw ^ {lds_swizzle(m * T)}] = bload(rl, lo * 8u, {int(hi[m]) * 8}u);")
+    if not adjoint and fresh:
+        L.append("  }")
     L.append("  __syncthreads();")
     regs = ("vp", "vl") if adjoint else ("v",)
     for s in range(nst):
@@ -251,9 +269,11 @@ def program_source(enc, n_fwd, n_adj, seed_masks=None):
     offs = list(enc.fwd_offsets) + list(enc.adj_offsets) + [len(enc.prog)]
     starts = sorted(set(offs))
     ends = {o: starts[i + 1] for i, o in enumerate(starts[:-1])}
+    seen = 0
     for i in range(n_fwd):
         o = enc.fwd_offsets[i]
-        parts.append(pass_kernel_source(f"f{i}", enc.prog[o:ends[o]], False))
+        parts.append(pass_kernel_source(f"f{i}", enc.prog[o:ends[o]], False, fresh=enc.tile_masks[i] & ~seen))
+        seen |= enc.tile_masks[i]
     for i in range(n_adj):
         o = enc.adj_offsets[i]
         parts.append(pass_kernel_source(f"a{i}", enc.prog[o:ends[o]], True))

@@ -167,16 +167,6 @@ class _Program:
                 raise RuntimeError("gradient slots of a pass are not contiguous")
             self.pass_gslots.append((min(gs), max(gs) + 1) if gs else (0, 0))
 
-    def checkpoint_fill(self, from_zero):
-        """For the out-of-place forward (forward_passes' ``outs``): whether each output buffer of passes
-        1 .. P-1 must start as zeros.  A pass whose tile holds bits that no earlier pass held reads zeros
-        there which the previous pass, skipping the tiles that hold zeros, never wrote."""
-        seen, fresh = 0, []
-        for mask in self.enc.tile_masks:
-            fresh.append(from_zero and bool(mask & ~seen))
-            seen |= mask
-        return [i + 1 < len(fresh) and fresh[i + 1] for i in range(1, len(fresh))]
-
     def enable_jit(self, device, forward=True, adjoint=True, seed_masks=None):
         """Compile per-pass kernels for this program on ``device`` (hiprtc); returns True if any are in use.
 
@@ -256,10 +246,13 @@ class _Program:
         return mats, amats
 
     def forward_passes(self, psi, mats, probe=None, from_zero=False, outs=None):
-        """Apply the passes to psi in place.  ``from_zero``: psi holds |0..0> per row, so each pass skips the
-        tiles that hold zeros (zero_masks).  ``outs`` (generated kernels only): one state buffer per pass
-        after the first; pass i >= 1 reads the previous output and writes outs[i] (out of place), so
-        psi and outs[1:] end up holding the state after every pass (checkpoints)."""
+        """Apply the passes to psi in place.  ``from_zero``: the sweep starts from |0..0>, so each pass skips
+        the tiles that hold zeros (zero_masks); generated kernels also take the amplitudes at tile bits no
+        earlier pass held as zero and the first pass builds |0..0> itself (kPassFromZero), so psi then
+        needs no initialisation (the interpreted kernels read it: psi must hold |0..0>).  ``outs``
+        (generated kernels only): one state buffer per pass after the first; pass i >= 1 reads the
+        previous output and writes outs[i] (out of place), so psi and outs[1:] end up holding the state
+        after every pass (checkpoints)."""
         dev = psi.device
         buf = self.buffers(dev)
         B = psi.shape[0]
@@ -273,9 +266,9 @@ class _Program:
             if probe:
                 probe.begin("sv_pass_fwd")
             if fns:
-                src, dst, flags = psi, None, 0
+                src, dst, flags = psi, None, 4 if from_zero else 0
                 if outs is not None and i > 0:
-                    src, dst, flags = outs[i - 1], outs[i], 1
+                    src, dst, flags = outs[i - 1], outs[i], flags | 1
                 hip.call("tedq_jit_pass", fns[i], 0, hip.ptr(src), hip.ptr(dst), B, self.n, self.plan.k, self.plan.r,
                          mask, zero, hip.ptr(mats), self.plan.mat_stride, None, 0, 0, None, flags, st)
             else:
@@ -506,14 +499,15 @@ class SVExecutor:
         psi = torch.empty((B, 2 ** self.n), dtype=torch.complex64, device=dev)
         st = hip.stream_handle(dev)
         _, init = self._zmask_buffer(dev)
-        if init is None:
-            hip.call("tedq_sv_init_zero", hip.ptr(psi), B, self.n, st)
-        else:
+        if init is not None:
             hip.call("tedq_sv_init_copy", hip.ptr(psi), hip.ptr(init), B, self.n, st)
+        elif self.fwd._jit_fns(dev, False) is None or self.fwd.zero_masks[-1]:  # pylint: disable=protected-access
+            # interpreted kernels read |0..0>; with generated ones the last pass writes every tile unless it
+            # skips some, and what it skips is read later (measurements) as zeros
+            hip.call("tedq_sv_init_zero", hip.ptr(psi), B, self.n, st)
         outs = None
         if need_grad and self._checkpoints_fit(dev, B, init is None):
-            fill = self.fwd.checkpoint_fill(init is None)
-            outs = [psi] + [torch.zeros_like(psi) if z else torch.empty_like(psi) for z in fill]
+            outs = [psi] + [torch.empty_like(psi) for _ in range(len(self.fwd.plan.passes) - 1)]
         self.fwd.forward_passes(psi, mats, self.probe, from_zero=init is None, outs=outs)
         if outs is None:
             return psi

@@ -192,19 +192,29 @@ def _apply_ops(regs_list, ops, mats, acc):
                 raise ValueError(f"unknown op {typ}")
 
 
-def run_pass(words, n, states, mats, acc, zero=0):
+def run_pass(words, n, states, mats, acc, zero=0, fresh=0):
     """Execute one encoded pass over each state vector in ``states`` (in place); tiles whose non-tile bits
-    meet ``zero`` are left untouched (the kernels' zero mask)."""
+    meet ``zero`` are left untouched (the kernels' zero mask).  ``fresh`` (a forward sweep from |0..0>,
+    kPassFromZero): amplitudes at local positions meeting these tile bits are taken as zero instead of
+    read, and when every tile bit is fresh the tile is |0..0> (nothing read)."""
     k, r, nst = int(words[0]), int(words[1]), int(words[2])
     tile_bits = [int(b) for b in words[P.HDR_TILE_BITS:P.HDR_TILE_BITS + k]]
     rest = [b for b in range(n) if b not in tile_bits]
     R = 1 << r
     local_phys = np.array([_pdep(j, tile_bits) for j in range(1 << k)], dtype=np.int64)
+    first = fresh != 0 and fresh == sum(1 << b for b in tile_bits)
     for t in range(1 << (n - k)):
         base = _pdep(t, rest)
         if base & zero:
             continue
         tiles = [s[base + local_phys].copy() for s in states]
+        if first:
+            for tile in tiles:
+                tile[:] = 0
+                tile[0] = 1
+        elif fresh:
+            for tile in tiles:
+                tile[(local_phys & fresh) != 0] = 0
         for si in range(nst):
             st = words[P.HDR_STAGES + P.STAGE_WORDS * si:P.HDR_STAGES + P.STAGE_WORDS * (si + 1)]
             nops, ostart, nonreg = int(st[0]), int(st[1]), int(st[2])
@@ -221,9 +231,9 @@ def run_pass(words, n, states, mats, acc, zero=0):
             s[base + local_phys] = tile
 
 
-def run_pass_out(words, n, src, dst, mats, zero=0):
+def run_pass_out(words, n, src, dst, mats, zero=0, fresh=0):
     """One forward pass read from ``src`` and written to ``dst`` (out of place), tiles meeting ``zero``
-    skipped (neither read nor written)."""
+    skipped (neither read nor written); ``fresh`` as in run_pass."""
     k = int(words[0])
     tile_bits = [int(b) for b in words[P.HDR_TILE_BITS:P.HDR_TILE_BITS + k]]
     rest = [b for b in range(n) if b not in tile_bits]
@@ -234,7 +244,7 @@ def run_pass_out(words, n, src, dst, mats, zero=0):
         if base & zero:
             continue
         work[base + local_phys] = src[base + local_phys]
-    run_pass(words, n, [work], mats, {}, zero)
+    run_pass(words, n, [work], mats, {}, zero, fresh)
     for t in range(1 << (n - k)):
         base = _pdep(t, rest)
         if base & zero:

@@ -372,10 +372,11 @@ def test_passes_skip_tiles_that_hold_zeros(seed, n, k, r):
 
 @pytest.mark.parametrize("seed,n,k,r", [(40, 8, 4, 2), (41, 9, 5, 3), (42, 10, 5, 2)])
 def test_checkpointed_forward_buffers(seed, n, k, r):
-    """Out-of-place forward with tile skipping (the checkpoints the adjoint reads): with the output buffers
-    the executor zero-fills (checkpoint_fill) and the rest uninitialised (NaN here), every checkpoint
-    equals the in-place state after that pass on the tiles the adjoint reads (its live tiles), and the
-    last one everywhere."""
+    """Out-of-place forward from |0..0> with tile skipping, as the generated kernels run it under
+    kPassFromZero (amplitudes at tile bits no earlier pass held taken as zero, the first tile built
+    without reading): with every buffer uninitialised (NaN here), psi included, each checkpoint equals
+    the in-place state after that pass on the tiles the adjoint reads (its live tiles), and the last one
+    everywhere when the last pass skips nothing.  Reading the fresh positions instead fails."""
     from tedq_amd.engine.sv import _Program
     rng = np.random.default_rng(seed)
     spec = random_spec(n, 40, rng)
@@ -385,33 +386,41 @@ def test_checkpointed_forward_buffers(seed, n, k, r):
     nq, units, Pn, pool = lower_circuit(circ.compilecircuit(backend="pytorch"))
     prog = _Program(nq, units, Pn, pool, k, r, specialize=True)
     P = len(prog.plan.passes)
-    fill = prog.checkpoint_fill(True)
-    assert len(fill) == P - 1
     mats, _ = EMU.build_tables(prog.enc, prog.plan, values)
-    # in place, every tile: the states after each pass
+    # in place, every tile, from an initialised |0..0>: the states after each pass
     ref, states = np.zeros(2 ** n, dtype=np.complex128), []
     ref[0] = 1
     for off in prog.enc.fwd_offsets:
         EMU.run_pass(prog.enc.prog[off:], n, [ref], mats, {})
         states.append(ref.copy())
-    # out of place with skipping
-    psi = np.zeros(2 ** n, dtype=np.complex128)
-    psi[0] = 1
-    outs = [psi] + [np.zeros(2 ** n, dtype=np.complex128) if z else np.full(2 ** n, np.nan, dtype=np.complex128)
-                    for z in fill]
-    for i, off in enumerate(prog.enc.fwd_offsets):
-        if i == 0:
-            EMU.run_pass(prog.enc.prog[off:], n, [psi], mats, {}, prog.zero_masks[0])
+    seen, fresh = 0, []
+    for mask in prog.enc.tile_masks:
+        fresh.append(mask & ~seen)
+        seen |= mask
+    assert fresh[0] == prog.enc.tile_masks[0]
+    idx = np.arange(2 ** n)
+    # the kernels' rule on every pass, then (a planted mistake) on the first pass only
+    for later in (True, False):
+        outs = [np.full(2 ** n, np.nan, dtype=np.complex128) for _ in range(P)]
+        inplace = np.full(2 ** n, np.nan, dtype=np.complex128)
+        for i, off in enumerate(prog.enc.fwd_offsets):
+            f = fresh[i] if (later or i == 0) else 0
+            if i == 0:
+                EMU.run_pass(prog.enc.prog[off:], n, [outs[0]], mats, {}, prog.zero_masks[0], f)
+            else:
+                EMU.run_pass_out(prog.enc.prog[off:], n, outs[i - 1], outs[i], mats, prog.zero_masks[i], f)
+            EMU.run_pass(prog.enc.prog[off:], n, [inplace], mats, {}, prog.zero_masks[i], f)
+        good = True
+        for i in range(P):
+            live = (idx & prog.zero_masks[i]) == 0
+            good &= np.allclose(outs[i][live], states[i][live], atol=1e-12)
+        live = (idx & prog.zero_masks[-1]) == 0
+        good &= np.allclose(inplace[live], states[-1][live], atol=1e-12)
+        if later:
+            assert good
+            assert prog.zero_masks[-1] != 0 or np.allclose(outs[-1], states[-1], atol=1e-12)
         else:
-            EMU.run_pass_out(prog.enc.prog[off:], n, outs[i - 1], outs[i], mats, prog.zero_masks[i])
-    for i in range(P):
-        words = prog.enc.prog[prog.enc.fwd_offsets[i]:]
-        kk = int(words[0])
-        tile = [int(b) for b in words[4:4 + kk]]
-        idx = np.arange(2 ** n)
-        live = (idx & prog.zero_masks[i]) == 0
-        assert np.allclose(outs[i][live], states[i][live], atol=1e-12), i
-    assert prog.zero_masks[-1] != 0 or np.allclose(outs[-1], states[-1], atol=1e-12)
+            assert good == (not any(fresh[1:]))
 
 
 @pytest.mark.parametrize("seed", [0, 1])
