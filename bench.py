@@ -11,9 +11,13 @@ Extra fields on the JSON line:
 * roofline: the dominant pass kernel, algorithmic bytes per launch (16 B per
   amplitude for a forward pass, 32 B for an adjoint pass over psi and lambda,
   24 B when it reads psi from the forward's checkpoints, over the tiles the pass
-  processes: from |0..0> the first passes skip the tiles that hold zeros)
+  processes: from |0..0> the first passes skip the tiles that hold zeros, and a
+  generated forward pass reads only amplitudes whose tile bits earlier passes
+  held)
   divided by its mean launch time, measured with events on the launch stream
-  during the timed steps, against 8 TB/s;
+  during the timed steps, against 8 TB/s; traffic (FETCH_SIZE + WRITE_SIZE)
+  and the VALU issue share come from the committed counter runs of the same
+  shapes;
 * cpu_baseline: the complex64 CPU restatement of the reference algorithm
   (tensordot + permute per gate, torch autograd backward) on the host cores,
   on a bounded sample of the same workload;
@@ -378,6 +382,26 @@ def measured_traffic(kernel, n, layers, batch, ex):
     return None
 
 
+def measured_valu_share(kernel, n, layers, batch, ex):
+    """Time-weighted VALU issue share of ``kernel`` (SQ_INSTS_VALU x 4 cycles over the SIMDs' cycles) from the
+    committed SQ counter runs of this workload (profiles/sv_pass_sq_counters_hea20_b256.json), when the run
+    uses a profiled shape pair; else None.  A pass kernel whose HBM fraction is low because it is busy
+    issuing arithmetic shows it here."""
+    path = os.path.join(ROOT, "profiles", "sv_pass_sq_counters_hea20_b256.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        recs = json.load(f)
+    shape = (ex.fwd.plan.k, ex.fwd.plan.r, ex.bwd.plan.k, ex.bwd.plan.r)
+    ckpt = ex._checkpoints_fit(torch.device("cuda", 0), batch, True)  # pylint: disable=protected-access
+    for rec in recs:
+        w = rec["workload"]
+        if ((w["qubits"], w["layers"], w["batch"]) == (n, layers, batch) and w["forward_checkpoints"] == ckpt
+                and (w["tile_bits"], w["reg_bits"], w["tile_bits_bwd"], w["reg_bits_bwd"]) == shape):
+            return rec["valu_issue_share"].get(kernel)
+    return None
+
+
 def tn_leg(c4=True, c3=True):
     """Contraction mode: the 40-qubit disordered chain (forward, batch 32) and the 5x5 lattice (fwd + grad)."""
     from tedq_amd import workloads as W  # pylint: disable=import-outside-toplevel
@@ -576,14 +600,22 @@ def main():
             "sv_pass_adj": [2.0 ** -bin(z).count("1") for z in ex.bwd.zero_masks]}
     # with per-pass checkpoints the adjoint reads psi and lambda and writes lambda only (24 B per amplitude)
     ckpt = ex._checkpoints_fit(dev, B, True)  # pylint: disable=protected-access
-    for kind, bpa in (("sv_pass_fwd", 16), ("sv_pass_adj", 24 if ckpt else 32)):
+    # generated forward passes from |0..0> read only the amplitudes whose tile bits earlier passes held
+    # (the first pass reads nothing): 8 B written plus 8 B x that fraction read per amplitude
+    fresh_loads = ex.fwd._jit_fns(dev, False) is not None  # pylint: disable=protected-access
+    per_pass = {"sv_pass_fwd": [], "sv_pass_adj": [(24 if ckpt else 32) * amps * f for f in live["sv_pass_adj"]]}
+    seen = 0
+    for i, (f, m) in enumerate(zip(live["sv_pass_fwd"], ex.fwd.enc.tile_masks)):
+        rd = (0.0 if i == 0 else 2.0 ** -bin(m & ~seen).count("1")) if fresh_loads else 1.0
+        per_pass["sv_pass_fwd"].append(8 * amps * f * (1 + rd))
+        seen |= m
+    for kind in ("sv_pass_fwd", "sv_pass_adj"):
         ms, cnt = probe.mean_ms(kind)
         if ms:
             frac = float(np.mean(live[kind]))
-            gbs = bpa * amps * frac / (ms * 1e-3) / 1e9
-            kernels[kind] = {"mean_ms": ms, "launches": cnt, "total_ms": ms * cnt,
-                             "bytes_per_launch": bpa * amps * frac, "live_tile_fraction": frac,
-                             "achieved_GBs": gbs}
+            nbytes = float(np.mean(per_pass[kind]))
+            kernels[kind] = {"mean_ms": ms, "launches": cnt, "total_ms": ms * cnt, "bytes_per_launch": nbytes,
+                             "live_tile_fraction": frac, "achieved_GBs": nbytes / (ms * 1e-3) / 1e9}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     kd = kernels[dom]
     line = {
@@ -606,6 +638,7 @@ def main():
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dom, n, L, B, ex),
+                     "valu_issue_share": measured_valu_share(dom, n, L, B, ex),
                      "bytes_per_launch": kd["bytes_per_launch"], "mean_launch_ms": kd["mean_ms"]},
         "kernels": kernels,
         "norm_check": {"max_abs_deviation": norm_dev, "rows": B,
