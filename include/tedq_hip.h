@@ -164,6 +164,10 @@ int tedq_jit_compile(const char* src, int n_headers, const char* const* header_s
                      const char* const* header_names, void** module_out);
 /* Kernel handle of `name` in a compiled module. */
 int tedq_jit_function(void* module, const char* name, void** fn_out);
+/* Per-thread scratch (private segment) bytes of a compiled kernel: nonzero when
+ * the compiler spilled registers, so the caller can compile that pass in
+ * another form.  No reference counterpart (a property of generated code). */
+int tedq_jit_function_scratch(void* fn, int* bytes_out);
 /* Unload a compiled module. */
 int tedq_jit_unload(void* module);
 /* Launch one generated pass (forward when adjoint == 0) over every circuit;
@@ -178,7 +182,9 @@ int tedq_jit_unload(void* module);
  * with the zero mask of a sweep from |0..0>) takes the amplitudes at tile bits
  * no earlier pass held as zero instead of reading them, and the first pass
  * builds the |0..0> tile without reading psi, so psi needs no initialisation
- * when the last pass writes every tile.  0: in place. */
+ * when the last pass writes every tile; 8 (adjoint, kernels generated to loop
+ * over tiles) launches only as many workgroups as the device holds at once,
+ * each loading its next tile while it processes the current one.  0: in place. */
 int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_t batch, int n_qubits, int tile_bits,
                   int reg_bits, uint64_t tile_mask, uint64_t zero_mask, const void* mats, int64_t mat_stride,
                   void* partials, int64_t partial_rows, int n_gslots, const double* seed, int flags, void* stream);

@@ -17,6 +17,8 @@
 // code memory; launches allocate nothing.
 #include <hip/hiprtc.h>
 
+#include <algorithm>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -68,6 +70,18 @@ extern "C" int tedq_jit_compile(const char* src, int n_headers, const char* cons
   return kOk;
 }
 
+// Compute units of the current device, queried once per device.
+static int device_cu_count() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  int c = cache[dev].load(std::memory_order_relaxed);
+  if (c > 0) return c;
+  if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) return 0;
+  cache[dev].store(c, std::memory_order_relaxed);
+  return c;
+}
+
 extern "C" int tedq_jit_function(void* module, const char* name, void** fn_out) {
   if (module == nullptr || name == nullptr || fn_out == nullptr) {
     set_error("tedq_jit_function: bad arguments");
@@ -80,6 +94,21 @@ extern "C" int tedq_jit_function(void* module, const char* name, void** fn_out) 
     return kBadArgument;
   }
   *fn_out = f;
+  return kOk;
+}
+
+extern "C" int tedq_jit_function_scratch(void* fn, int* bytes_out) {
+  if (fn == nullptr || bytes_out == nullptr) {
+    set_error("tedq_jit_function_scratch: bad arguments");
+    return kBadArgument;
+  }
+  int v = 0;
+  const hipError_t e = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, static_cast<hipFunction_t>(fn));
+  if (e != hipSuccess) {
+    set_error("tedq_jit_function_scratch: %s", hipGetErrorString(e));
+    return kLaunchFailed;
+  }
+  *bytes_out = v;
   return kOk;
 }
 
@@ -121,9 +150,9 @@ extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_
   a.np = partial_rows;
   a.n_gslots = n_gslots;
   a.flags = flags;
-  if ((flags & ~(kPassOutOfPlace | kPassPsiReadOnly | kPassFromZero)) != 0 ||
+  if ((flags & ~(kPassOutOfPlace | kPassPsiReadOnly | kPassFromZero | kPassPersistent)) != 0 ||
       (!adjoint && (flags & kPassOutOfPlace) && lam == nullptr) || (adjoint && (flags & (kPassOutOfPlace | kPassFromZero))) ||
-      (!adjoint && (flags & kPassPsiReadOnly))) {
+      (!adjoint && (flags & (kPassPsiReadOnly | kPassPersistent)))) {
     set_error("tedq_jit_pass: flags %d do not fit a %s pass", flags, adjoint ? "adjoint" : "forward");
     return kBadArgument;
   }
@@ -133,8 +162,19 @@ extern "C" int tedq_jit_pass(void* fn, int adjoint, void* psi, void* lam, int64_
     set_error("tedq_jit_pass: tile of %zu LDS bytes or %lld tiles is out of range", lds, (long long)a.total_tiles);
     return kBadArgument;
   }
+  unsigned grid = (unsigned)a.total_tiles;
+  if (flags & kPassPersistent) {
+    // as many workgroups as the CUs hold at once by LDS (the kernel loops over the rest)
+    const int cus = device_cu_count();
+    if (cus <= 0) {
+      set_error("tedq_jit_pass: cannot query the device's compute units");
+      return kLaunchFailed;
+    }
+    const int64_t resident = (int64_t)cus * (int64_t)std::max<size_t>(1, (160 * 1024) / lds);
+    grid = (unsigned)std::min<int64_t>(a.total_tiles, resident);
+  }
   void* args[] = {&psi, &lam, const_cast<void**>(&mats), &partials, &a, const_cast<double**>(&seed)};
-  const hipError_t e = hipModuleLaunchKernel(static_cast<hipFunction_t>(fn), (unsigned)a.total_tiles, 1, 1, T, 1, 1,
+  const hipError_t e = hipModuleLaunchKernel(static_cast<hipFunction_t>(fn), grid, 1, 1, T, 1, 1,
                                              (unsigned)lds, static_cast<hipStream_t>(stream), args, nullptr);
   if (e != hipSuccess) {
     set_error("tedq_jit_pass: launch failed: %s", hipGetErrorString(e));

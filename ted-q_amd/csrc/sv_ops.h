@@ -45,7 +45,8 @@ struct PassArgs {
   int gT;                   // log2(tiles per workgroup)
   int64_t np;               // adjoint: partials are [batch][n_gslots][np][8] (partial rows per slot per circuit)
   int n_gslots;
-  int flags;                // generated passes: kPassOutOfPlace, kPassPsiReadOnly, kPassFromZero
+  int flags;                // generated passes: kPassOutOfPlace, kPassPsiReadOnly, kPassFromZero,
+                            // kPassPersistent
 };
 
 // Generated forward pass: read psi, write the result to the second state pointer (a checkpoint).
@@ -57,6 +58,10 @@ constexpr int kPassPsiReadOnly = 2;
 // earlier pass held are 0 wherever the state is nonzero, so those amplitudes are taken as zero instead of
 // loaded (the first pass, whose tile bits are all new, builds the |0..0> tile without reading psi).
 constexpr int kPassFromZero = 4;
+// Generated adjoint pass whose workgroups loop over tiles (blockIdx.x, + gridDim.x, ...), loading the next
+// tile into registers while processing the current one: launched with at most as many workgroups as the
+// device holds at once.
+constexpr int kPassPersistent = 8;
 
 // ---------------------------------------------------------------------------
 // register-level gate kernels.  B is the target register bit and CM a control

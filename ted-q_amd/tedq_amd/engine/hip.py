@@ -43,6 +43,7 @@ _SIGNATURES = {
     "tedq_sv_grad_reduce": ([c_vp, c_i64, c_int, c_i64, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_vp], c_int),
     "tedq_jit_compile": ([ctypes.c_char_p, c_int, c_vp, c_vp, c_vp], c_int),
     "tedq_jit_function": ([c_vp, ctypes.c_char_p, c_vp], c_int),
+    "tedq_jit_function_scratch": ([c_vp, c_vp], c_int),
     "tedq_jit_unload": ([c_vp], c_int),
     "tedq_jit_pass": ([c_vp, c_int, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_u64, c_u64, c_vp, c_i64, c_vp, c_i64,
                        c_int, c_vp, c_int, c_vp], c_int),

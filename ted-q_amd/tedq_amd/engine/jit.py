@@ -29,6 +29,20 @@ _CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abs
 _HEADERS = ("tedq_common.h", "sv_ops.h")
 
 
+# generated adjoint kernels loop over tiles with a register prefetch of the next tile (kPassPersistent)
+# where one workgroup's LDS fills a CU (see persistent_pays)
+PERSISTENT_ADJOINT = True
+
+
+def persistent_pays(k, adjoint):
+    """Whether generated kernels of this tile size loop over tiles with a register prefetch: only where
+    one workgroup's LDS fills a CU (an adjoint tile of 2^13 amplitudes or more takes 128 KiB of the 160).
+    There the load of a tile otherwise waits with nothing to overlap it (20q ansatz, adjoint 2^13:
+    30.5 -> 27.4 ms); where two workgroups share a CU they overlap each other and the loop only costs
+    registers (adjoint 2^12: 30.3 -> 32.1 ms)."""
+    return PERSISTENT_ADJOINT and adjoint and (1 << k) * 16 > 80 * 1024
+
+
 def supported(n, k, r, adjoint):
     lds = (1 << k) * 8 * (2 if adjoint else 1)
     return k < n <= 28 and lds <= 160 * 1024 and k - r <= 9 and 1 <= r <= 5
@@ -160,7 +174,7 @@ def _seed_lines(hi, R, masks):
     return out
 
 
-def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0):
+def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0, persistent=False):
     """HIP source of one generated pass kernel from its program words (program.py layout).
 
     ``seed_masks`` (adjoint only): instead of loading lambda, form it from the loaded psi as
@@ -168,7 +182,11 @@ def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0):
     expectation values -- for the first pass of the sweep.
     ``fresh`` (forward only): the tile bits no earlier pass held.  Under kPassFromZero (a sweep from
     |0..0>) the state is zero wherever one of them is 1, so those amplitudes are set instead of loaded;
-    when every tile bit is fresh (the first pass) the tile is |0..0> itself and nothing is loaded."""
+    when every tile bit is fresh (the first pass) the tile is |0..0> itself and nothing is loaded.
+    ``persistent`` (adjoint only; launched with kPassPersistent): each workgroup loops over the tiles
+    blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next tile's psi and lambda into registers
+    while the current one is processed, so the load of a tile overlaps the arithmetic of the previous
+    one even where a single workgroup fills a CU's LDS."""
     words = np.asarray(words, dtype=np.int32)
     k, r, nst, tT = (int(x) for x in words[:4])
     R, T = 1 << r, 1 << tT
@@ -176,32 +194,58 @@ def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0):
     hi = words[HDR_HI_OFF:HDR_HI_OFF + 64].view(np.int64)
     seg = min(T, 64)
     W = max(T >> 6, 1)
+    persistent = persistent and adjoint
+    seeded = adjoint and seed_masks is not None
+    tile = "TILE" if persistent else "(int64_t)blockIdx.x"
     L = [f'extern "C" __global__ void __launch_bounds__({T}) {name}(',
          "    float2* __restrict__ psi, float2* __restrict__ lam, const float2* __restrict__ mats,",
          "    float* __restrict__ partials, tedq::PassArgs a, const double* __restrict__ seed) {",
          "  using namespace tedq;",
-         "  extern __shared__ float2 lds[];",
-         "  const int tid = threadIdx.x;",
-         "  int64_t circ, gbase;",
-         "  tile_origin(a, blockIdx.x, &circ, &gbase);",
-         "  const float2* __restrict__ mb = mats + circ * a.mat_stride;",
-         f"  const uint32_t lo = {_pdep_expr('(uint32_t)tid', tile_bits[:tT])};",
-         "  const int sw = swz(tid);",
-         "  const __amdgpu_buffer_rsrc_t rp = make_rsrc(psi + gbase, (uint32_t)(8u << a.n));"]
+         "  extern __shared__ float2 lds[];"]
+    thread_lines = [f"  const uint32_t lo = {_pdep_expr('(uint32_t)tid', tile_bits[:tT])};",
+                    "  const int sw = swz(tid);"]
+    if adjoint:
+        thread_lines += [f"  float2* ldl = lds + {T * R};", "  const int lane = tid & 63;", "  (void)lane;"]
+    if not persistent:
+        L += ["  const int tid = threadIdx.x;"] + thread_lines
+    else:
+        # registers holding the next tile (psi, and lambda unless the seed forms it from psi)
+        regs_in = [("qp", "psi")] + ([] if seeded else [("ql", "lam")])
+
+        def prefetch(expr, lo_var):
+            out = [f"  {{ int64_t c_, g_; tile_origin(a, {expr}, &c_, &g_); (void)c_;"]
+            for v, buf in regs_in:
+                out.append(f"    const __amdgpu_buffer_rsrc_t f{v} = make_rsrc({buf} + g_, (uint32_t)(8u << a.n));")
+                out += [f"    {v}[{m}] = bload(f{v}, {lo_var} * 8u, {int(hi[m]) * 8}u);" for m in range(R)]
+            return out + ["  }"]
+        L += [f"  float2 {v}[{R}];" for v, _ in regs_in]
+        L.append(f"  const uint32_t lo0 = {_pdep_expr('(uint32_t)threadIdx.x', tile_bits[:tT])};")
+        L += prefetch("(int64_t)blockIdx.x", "lo0")
+        L.append("  for (int64_t TILE = blockIdx.x; TILE < a.total_tiles; TILE += gridDim.x) {")
+        # per-thread offsets are recomputed in every iteration: hoisted out of the loop, the LDS offsets of
+        # all stages would stay live across it (about 90 more VGPRs, and spills with the prefetch)
+        L += ["  int tid = threadIdx.x;", '  asm volatile("" : "+v"(tid));'] + thread_lines
+    L += ["  int64_t circ, gbase;",
+          f"  tile_origin(a, {tile}, &circ, &gbase);",
+          "  const float2* __restrict__ mb = mats + circ * a.mat_stride;",
+          "  const __amdgpu_buffer_rsrc_t rp = make_rsrc(psi + gbase, (uint32_t)(8u << a.n));"]
     if adjoint:
         L += ["  const __amdgpu_buffer_rsrc_t rl = make_rsrc(lam + gbase, (uint32_t)(8u << a.n));",
-              f"  float2* ldl = lds + {T * R};",
-              "  const int lane = tid & 63;",
-              f"  const int64_t prow = ((int64_t)blockIdx.x & ((1ll << tiles_log2(a)) - 1)) * {W} + (tid >> 6);",
+              f"  const int64_t prow = ({tile} & ((1ll << tiles_log2(a)) - 1)) * {W} + (tid >> 6);",
               "  float* const pbase = partials + (circ * a.n_gslots * a.np + prow) * 8;",
               "  float sacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};",
-              "  (void)lane; (void)sacc;"]
-    if adjoint and seed_masks is not None:
+              "  (void)sacc;"]
+    if seeded:
         L += _seed_lines(hi, R, seed_masks)
         for m in range(R):
             x = lds_swizzle(m * T)
-            L.append(f"  {{ const float2 v = bload(rp, lo * 8u, {int(hi[m]) * 8}u); lds[sw ^ {x}] = v;"
+            src = f"qp[{m}]" if persistent else f"bload(rp, lo * 8u, {int(hi[m]) * 8}u)"
+            L.append(f"  {{ const float2 v = {src}; lds[sw ^ {x}] = v;"
                      f" ldl[sw ^ {x}] = make_float2(d{m} * v.x, d{m} * v.y); }}")
+    elif persistent:
+        for m in range(R):
+            x = lds_swizzle(m * T)
+            L.append(f"  lds[sw ^ {x}] = qp[{m}]; ldl[sw ^ {x}] = ql[{m}];")
     L.append("  (void)seed;")
     tile_mask = sum(1 << b for b in tile_bits)
     if not adjoint and fresh:
@@ -216,13 +260,17 @@ def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0):
                 L.append(f"    {{ float2 v = make_float2(0.f, 0.f); if ((lo & {fresh}u) == 0u) v = bload(rp, lo * 8u, "
                          f"{int(hi[m]) * 8}u); lds[sw ^ {x}] = v; }}")
         L.append("  } else {")
-    for m in range(R if seed_masks is None or not adjoint else 0):
+    for m in range(0 if seeded or persistent else R):
         L.append(f"  lds[sw ^ {lds_swizzle(m * T)}] = bload(rp, lo * 8u, {int(hi[m]) * 8}u);")
         if adjoint:
             L.append(f"  ldl[sw ^ {lds_swizzle(m * T)}] = bload(rl, lo * 8u, {int(hi[m]) * 8}u);")
     if not adjoint and fresh:
         L.append("  }")
     L.append("  __syncthreads();")
+    if persistent:
+        # the next tile's loads are in flight during this tile's stages
+        L.append("  if (TILE + gridDim.x < a.total_tiles)")
+        L += prefetch("TILE + gridDim.x", "lo")
     regs = ("vp", "vl") if adjoint else ("v",)
     for s in range(nst):
         base = HDR_STAGES + STAGE_WORDS * s
@@ -252,6 +300,9 @@ def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0):
         L.append("  }")
         for m in range(R):
             L.append(f"  bstore(ldl[sw ^ {lds_swizzle(m * T)}], rl, lo * 8u, {int(hi[m]) * 8}u);")
+        if persistent:
+            # this tile's stores read LDS, which the next tile's first writes overwrite
+            L += ["  __syncthreads();", "  }"]
     else:
         # kPassOutOfPlace: the result goes to the second state pointer (a checkpoint), psi keeps its input
         L.append("  const __amdgpu_buffer_rsrc_t ro = make_rsrc(((a.flags & kPassOutOfPlace) ? lam : psi) + gbase,"
@@ -262,9 +313,11 @@ def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0):
     return "\n".join(L)
 
 
-def program_source(enc, n_fwd, n_adj, seed_masks=None):
+def program_source(enc, n_fwd, n_adj, seed_masks=None, persistent=False, only=None):
     """Generated source of every pass of an encoded plan: kernels f0.. (forward) and a0.. (adjoint), and
-    with ``seed_masks`` also a0s, the first adjoint pass forming lambda from psi (pass_kernel_source)."""
+    with ``seed_masks`` also a0s, the first adjoint pass forming lambda from psi (pass_kernel_source);
+    ``persistent``: the adjoint kernels loop over tiles with a register prefetch (kPassPersistent);
+    ``only``: the kernel names to emit (default all)."""
     parts = ['#include "sv_ops.h"', ""]
     offs = list(enc.fwd_offsets) + list(enc.adj_offsets) + [len(enc.prog)]
     starts = sorted(set(offs))
@@ -272,15 +325,24 @@ def program_source(enc, n_fwd, n_adj, seed_masks=None):
     seen = 0
     for i in range(n_fwd):
         o = enc.fwd_offsets[i]
-        parts.append(pass_kernel_source(f"f{i}", enc.prog[o:ends[o]], False, fresh=enc.tile_masks[i] & ~seen))
+        if only is None or f"f{i}" in only:
+            parts.append(pass_kernel_source(f"f{i}", enc.prog[o:ends[o]], False, fresh=enc.tile_masks[i] & ~seen))
         seen |= enc.tile_masks[i]
     for i in range(n_adj):
         o = enc.adj_offsets[i]
-        parts.append(pass_kernel_source(f"a{i}", enc.prog[o:ends[o]], True))
-    if seed_masks is not None and n_adj:
+        if only is None or f"a{i}" in only:
+            parts.append(pass_kernel_source(f"a{i}", enc.prog[o:ends[o]], True, persistent=persistent))
+    if seed_masks is not None and n_adj and (only is None or "a0s" in only):
         o = enc.adj_offsets[0]
-        parts.append(pass_kernel_source("a0s", enc.prog[o:ends[o]], True, seed_masks))
+        parts.append(pass_kernel_source("a0s", enc.prog[o:ends[o]], True, seed_masks, persistent=persistent))
     return "\n\n".join(parts) + "\n"
+
+
+def scratch_bytes(fn):
+    """Per-thread scratch bytes of a compiled kernel (nonzero: the compiler spilled registers)."""
+    out = ctypes.c_int(0)
+    hip.call("tedq_jit_function_scratch", fn, ctypes.byref(out))
+    return out.value
 
 
 class JitModule:

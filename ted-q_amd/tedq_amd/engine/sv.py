@@ -185,13 +185,27 @@ class _Program:
             offs = self.enc.adj_offsets if adj else self.enc.fwd_offsets
             names = [f"{kind}{i}" for i in range(len(offs))]
             masks = seed_masks if adj and offs else None
-            src = jit.program_source(self.enc, 0 if adj else len(offs), len(offs) if adj else 0, masks)
+            all_names = names + (["a0s"] if masks is not None else [])
+            n_f, n_a = (0, len(offs)) if adj else (len(offs), 0)
+            persistent = jit.persistent_pays(k, adj)
             with torch.cuda.device(device):
-                mod = jit.JitModule(src, names + (["a0s"] if masks is not None else []))
+                mod = jit.JitModule(jit.program_source(self.enc, n_f, n_a, masks, persistent=persistent), all_names)
+                fns = dict(mod.functions)
+                plain = set()
+                if persistent:
+                    # a kernel that spills with the prefetch registers runs in the one-tile-per-workgroup form
+                    plain = {nm for nm in all_names if jit.scratch_bytes(fns[nm]) > 0}
+                    if plain:
+                        mod2 = jit.JitModule(jit.program_source(self.enc, n_f, n_a, masks, only=plain), sorted(plain))
+                        have["mods"].append(mod2)
+                        fns.update(mod2.functions)
             have["mods"].append(mod)
-            have[kind] = [mod.functions[nm] for nm in names]
+            have[kind] = [fns[nm] for nm in names]
+            # launch flag per generated kernel: kPassPersistent for those that loop over tiles
+            have.setdefault("flags", {}).update({fns[nm].value: (8 if persistent and nm not in plain else 0)
+                                                 for nm in all_names})
             if masks is not None:
-                have["a_seed"] = (tuple(masks), mod.functions["a0s"])
+                have["a_seed"] = (tuple(masks), fns["a0s"])
         return have["f"] is not None or have["a"] is not None
 
     def seed_fn(self, device, masks):
@@ -294,6 +308,8 @@ class _Program:
         fns = self._jit_fns(dev, True)
         if (seed is not None or ckpt is not None) and not fns:
             raise RuntimeError("a fused adjoint seed or checkpoints need the generated adjoint kernels")
+        # generated adjoint kernels that loop over tiles are launched with the resident grid only
+        kflags = self._jit[(dev.type, dev.index)].get("flags", {}) if fns else {}
         P = len(self.plan.passes)
         if from_zero:
             per_tile = rows >> (self.n - self.plan.k)
@@ -308,6 +324,7 @@ class _Program:
             if fns:
                 fn, dy = (seed[0], hip.ptr(seed[1])) if seed is not None and i == 0 else (fns[i], None)
                 src, flags = (ckpt[P - 1 - i], 2) if ckpt is not None else (psi, 0)
+                flags |= kflags.get(fn.value, 0)
                 hip.call("tedq_jit_pass", fn, 1, hip.ptr(src), hip.ptr(lam), B, self.n, self.plan.k, self.plan.r,
                          mask, zero, hip.ptr(mats), self.plan.mat_stride, hip.ptr(partials), rows, G, dy, flags,
                          st)

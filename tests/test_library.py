@@ -119,11 +119,14 @@ def test_generated_pass_kernels_compile_for_gfx950(tmp_path):
     ex = SVExecutor(n, units, P, pool, cc._measurements, k_max=6, r=3)  # pylint: disable=protected-access
     prog = ex.fwd
     assert jit.supported(n, prog.plan.k, prog.plan.r, True)
-    src = jit.program_source(prog.enc, len(prog.enc.fwd_offsets), len(prog.enc.adj_offsets))
-    path = tmp_path / "gen.hip"
-    path.write_text(src)
     csrc = os.path.join(ROOT, "ted-q_amd", "csrc")
-    out = subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
-                          "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", "-I", csrc, str(path), "-o",
-                          str(tmp_path / "gen.o")], capture_output=True, text=True, timeout=300)
-    assert out.returncode == 0, out.stderr[-2000:]
+    # the one-tile-per-workgroup kernels, and the adjoint kernels that loop over tiles with the fused seed
+    for tag, kw in (("plain", {}), ("persistent", {"seed_masks": tuple(ex.zmasks), "persistent": True})):
+        src = jit.program_source(prog.enc, len(prog.enc.fwd_offsets), len(prog.enc.adj_offsets), **kw)
+        assert ("TILE += gridDim.x" in src) == (tag == "persistent")
+        path = tmp_path / f"gen_{tag}.hip"
+        path.write_text(src)
+        out = subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                              "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", "-I", csrc, str(path), "-o",
+                              str(tmp_path / f"gen_{tag}.o")], capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr[-2000:]
