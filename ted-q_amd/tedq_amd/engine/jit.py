@@ -277,7 +277,10 @@ def pass_kernel_source(name, words, adjoint, seed_masks=None, fresh=0, persisten
         nops, op_off, nonreg = int(words[base]), int(words[base + 1]), int(words[base + 2])
         offs = [int(x) for x in words[base + 4:base + 4 + R]]
         L.append("  {")
-        L.append(f"    const int sb = swz({_pdep_expr('tid', _bits(nonreg))});")
+        # the thread index is re-read through an empty asm in every stage: otherwise the compiler computes
+        # the LDS offsets of all stages up front and keeps them live (up to ~100 more VGPRs in long passes)
+        L.append('    int ts = threadIdx.x; asm volatile("" : "+v"(ts));')
+        L.append(f"    const int sb = swz({_pdep_expr('ts', _bits(nonreg))});")
         for v in regs:
             L.append(f"    float2 {v}[{R}];")
         for m in range(R):
