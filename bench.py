@@ -392,12 +392,15 @@ def measured_valu_share(kernel, n, layers, batch, ex):
         return None
     with open(path) as f:
         recs = json.load(f)
-    shape = (ex.fwd.plan.k, ex.fwd.plan.r, ex.bwd.plan.k, ex.bwd.plan.r)
+    # a kernel family's share depends on its own tile shape (and, for the adjoint, on checkpoints)
+    adj = kernel == "sv_pass_adj"
+    shape = (ex.bwd.plan.k, ex.bwd.plan.r) if adj else (ex.fwd.plan.k, ex.fwd.plan.r)
     ckpt = ex._checkpoints_fit(torch.device("cuda", 0), batch, True)  # pylint: disable=protected-access
-    for rec in recs:
+    for rec in reversed(recs):
         w = rec["workload"]
-        if ((w["qubits"], w["layers"], w["batch"]) == (n, layers, batch) and w["forward_checkpoints"] == ckpt
-                and (w["tile_bits"], w["reg_bits"], w["tile_bits_bwd"], w["reg_bits_bwd"]) == shape):
+        own = (w["tile_bits_bwd"], w["reg_bits_bwd"]) if adj else (w["tile_bits"], w["reg_bits"])
+        if ((w["qubits"], w["layers"], w["batch"]) == (n, layers, batch) and own == shape
+                and (not adj or w["forward_checkpoints"] == ckpt)):
             return rec["valu_issue_share"].get(kernel)
     return None
 

@@ -108,11 +108,19 @@ __global__ void __launch_bounds__(kRedThreads) k_expval_z_wht(const float2* __re
   double acc[kMaxZ];
 #pragma unroll
   for (int m = 0; m < kMaxZ; ++m) acc[m] = 0.0;
+  // all 64 amplitudes of this thread are requested before any is used, so that a wave keeps 64 loads in
+  // flight (issued per quarter, the chunk's reads ran at 2.9 TB/s)
+  float2 av[4][16];
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) av[s][j] = st[base + (s << 12) + (j << 8) + t];
+#pragma unroll
   for (int s = 0; s < 4; ++s) {
     double w[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const float2 a = st[base + (s << 12) + (j << 8) + t];
+      const float2 a = av[s][j];
       w[j] = (double)a.x * a.x + (double)a.y * a.y;
     }
 #pragma unroll
