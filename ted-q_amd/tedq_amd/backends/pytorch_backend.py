@@ -68,10 +68,15 @@ class _AdjointFunction(torch.autograd.Function):
             psi_sink.append(psi)
         ctx.executor = executor
         ctx.save_for_backward(rows)
+        ctx.block = False
         if isinstance(psi, torch.Tensor):
             # the adjoint sweep walks psi back in place: the first backward consumes it (no copy), a
             # second backward through the same graph recomputes the forward
             ctx.psi = psi
+            if _scalar_outputs(executor):
+                # one [B, M] float32 block (one stack and one conversion instead of one per output)
+                ctx.block = True
+                return _scalar_block(outs)
         else:                               # contraction mode: a run record, recomputed in backward
             ctx.run = psi
         res = []
@@ -94,8 +99,21 @@ class _AdjointFunction(torch.autograd.Function):
         (rows,) = ctx.saved_tensors
         psi = ctx.psi if ctx.psi is not None else ex.run_forward(rows)
         ctx.psi = None
+        if ctx.block:
+            # columns of one float64 block (views, no copy per output)
+            grads = list(grads[0].to(torch.float64).unbind(1))
         grad = ex.adjoint(rows, psi, grads, grad_dtype=rows.dtype)
         return None, grad, None
+
+
+def _scalar_outputs(executor):
+    """Every measurement of a state-vector executor is an expectation value (one number per row)."""
+    return bool(executor.specs) and all(s.kind in ("zexp", "fexp") for s in executor.specs)
+
+
+def _scalar_block(outs):
+    """Per-measurement [B] expectation values as one [B, M] float32 tensor."""
+    return torch.stack(outs, 1).to(torch.float32)
 
 
 class TorchExecute(torch.autograd.Function):
@@ -257,6 +275,8 @@ class PyTorchBackend(CompiledCircuit):
         ex = self.executor()
         psi = ex.run_forward(rows)
         outs = ex.measure(psi)
+        if isinstance(psi, torch.Tensor) and _scalar_outputs(ex):
+            return [_scalar_block(outs)], psi
         res = []
         for spec, o in zip(ex.specs, outs):
             res.append(o.to(torch.float32) if spec.kind in ("zexp", "fexp") else o)
@@ -276,10 +296,15 @@ class PyTorchBackend(CompiledCircuit):
                 psi = ex.run_forward(rows)
                 sink.append(psi)
                 outs = ex.measure(psi)
-                outs = [o.to(torch.float32) if s.kind in ("zexp", "fexp") else o for s, o in zip(ex.specs, outs)]
-        results = [o[0] for o in outs]
+                if isinstance(psi, torch.Tensor) and _scalar_outputs(ex):
+                    outs = _scalar_block(outs)
+                else:
+                    outs = [o.to(torch.float32) if s.kind in ("zexp", "fexp") else o for s, o in zip(ex.specs, outs)]
         self._after_state(sink[0])
-        out = self._stack(results, False)
+        if isinstance(outs, torch.Tensor):          # [1, M] block of expectation values
+            out = outs[0]
+        else:
+            out = self._stack([o[0] for o in outs], False)
         home = self._device if self._device is not None else torch.device("cpu")
         return out.to(home) if out.device != home else out
 
@@ -303,7 +328,7 @@ class PyTorchBackend(CompiledCircuit):
         else:
             with torch.no_grad():
                 return self._run_rows(rows)[0][0]
-        return self._stack(list(outs), True)
+        return outs if isinstance(outs, torch.Tensor) else self._stack(list(outs), True)
 
     # ------------------------------------------------------------------
     def param_shift_execute(self, *params):
