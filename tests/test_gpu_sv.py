@@ -853,3 +853,35 @@ def test_checkpoints_with_initial_state_and_nonz_outputs(kind):
           f"vs in-place {np.max(np.abs(grads[0] - grads[1])):.2e}", flush=True)
     assert e_v < TOL and e_g < TOL
     assert np.max(np.abs(grads[0] - grads[1])) < 1e-5
+
+
+def test_persistent_adjoint_kernels_equal_the_one_tile_form():
+    """Generated adjoint kernels of 2^13-amplitude tiles loop over tiles with a register prefetch of the
+    next tile (kPassPersistent, resident grid); the same passes one tile per workgroup give bit-identical
+    gradients (checkpointed sweep with the fused seed, and the in-place sweep)."""
+    from tedq_amd import workloads as W
+    from tedq_amd.engine import jit
+    n, L, B = 16, 4, 5
+    rng = np.random.default_rng(151)
+    rows = torch.tensor(rng.uniform(0, 2 * np.pi, size=(B, 2 * n * L)), dtype=torch.float32, device=DEV)
+    ups = [torch.tensor(rng.normal(size=B), dtype=torch.float32, device=DEV) for _ in range(n)]
+    grads = {}
+    keep = jit.PERSISTENT_ADJOINT
+    try:
+        for flag in (True, False):
+            jit.PERSISTENT_ADJOINT = flag
+            ex = W.hea(n, L, jit=True, tile_bits=13, reg_bits=5, tile_bits_bwd=13, reg_bits_bwd=4).executor()
+            print(f"case persistent={flag}", flush=True)
+            psi = ex.run_forward(rows, need_grad=True)
+            assert getattr(psi, "tedq_checkpoints", None) is not None
+            g_ck = ex.adjoint(rows, psi, ups)
+            ex.checkpoint_bytes = 0
+            g_ip = ex.adjoint(rows, ex.run_forward(rows, need_grad=True), ups)
+            flags = ex.bwd._jit[(DEV.type, DEV.index)]["flags"]  # pylint: disable=protected-access
+            # (a kernel that would spill with the prefetch registers keeps the one-tile form)
+            assert any(v == 8 for v in flags.values()) if flag else all(v == 0 for v in flags.values())
+            grads[flag] = (g_ck.cpu().numpy(), g_ip.cpu().numpy())
+    finally:
+        jit.PERSISTENT_ADJOINT = keep
+    assert np.array_equal(grads[True][0], grads[False][0])
+    assert np.array_equal(grads[True][1], grads[False][1])
