@@ -885,3 +885,39 @@ def test_persistent_adjoint_kernels_equal_the_one_tile_form():
         jit.PERSISTENT_ADJOINT = keep
     assert np.array_equal(grads[True][0], grads[False][0])
     assert np.array_equal(grads[True][1], grads[False][1])
+
+
+@pytest.mark.parametrize("kind,tile", [("probs", 10), ("state", 10), ("probs", 6), ("state", 6)])
+def test_generated_forward_with_idle_qubits(kind, tile):
+    """Generated forward kernels from |0..0> skip loading amplitudes at tile bits no earlier pass held; when
+    some qubits carry no gate, no pass holds their bits and the last pass skips tiles, so the state is
+    still cleared first (the measurement reads it whole).  Probabilities (with gradients) and the state of
+    a 15-qubit circuit whose gates touch 7 qubits, in one pass and in several, against the oracle."""
+    n = 15
+    rng = np.random.default_rng(171)
+    spec = [(nm, [int(q) for q in qs], k) for nm, qs, k in _random_spec(7, 50, rng)]
+    npar = sum(s[2] for s in spec)
+    meas = (lambda: qai.probs(qubits=[0, 3, 14])) if kind == "probs" else (lambda: qai.state())  # noqa: E731
+    c, _ = _build(n, spec, meas, jit=True, tile_bits=tile, reg_bits=3, tile_bits_bwd=tile, reg_bits_bwd=3)
+    ex = c.executor()
+    vals = rng.uniform(-math.pi, math.pi, npar)
+    t = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+    out = c(t)
+    assert ex.fwd._jit_fns(DEV, False) is not None and ex.fwd.zero_masks[-1] != 0  # pylint: disable=protected-access
+    assert (len(ex.fwd.plan.passes) > 1) == (tile < 10)
+    print(f"case {kind}: passes {len(ex.fwd.plan.passes)}, last zero mask {ex.fwd.zero_masks[-1]:#x}", flush=True)
+    t64 = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    if kind == "probs":
+        dy = torch.tensor(np.random.default_rng(3).normal(size=tuple(out.shape)), dtype=torch.float32, device=DEV)
+        (out * dy).sum().backward()
+        ref = O.execute(n, _oracle_gates(spec, list(t64)), [("probs", [0, 3, 14])])
+        (ref * dy.double().cpu().reshape(ref.shape)).sum().backward()
+        e_v = relerr(out.detach().cpu().numpy().reshape(-1), ref.detach().numpy().reshape(-1))
+        e_g = relerr(t.grad.cpu().numpy(), t64.grad.numpy(), GFLOOR)
+        print(f"  values relerr {e_v:.2e}, gradients relerr {e_g:.2e}", flush=True)
+        assert e_v < TOL and e_g < TOL
+    else:
+        ref = O.propagate(n, _oracle_gates(spec, list(t64)), torch.complex128)
+        e_v = relerr(out.detach().cpu().numpy().reshape(-1), ref.detach().numpy().reshape(-1))
+        print(f"  state relerr {e_v:.2e}", flush=True)
+        assert e_v < TOL
