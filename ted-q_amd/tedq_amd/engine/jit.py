@@ -38,8 +38,8 @@ def persistent_pays(k, adjoint):
     """Whether generated kernels of this tile size loop over tiles with a register prefetch: only where
     one workgroup's LDS fills a CU (an adjoint tile of 2^13 amplitudes or more takes 128 KiB of the 160).
     There the load of a tile otherwise waits with nothing to overlap it (20q ansatz, adjoint 2^13:
-    30.5 -> 27.4 ms); where two workgroups share a CU they overlap each other and the loop only costs
-    registers (adjoint 2^12: 30.3 -> 32.1 ms)."""
+    30.5 -> 27.0 ms); where two workgroups share a CU they overlap each other and the loop only costs
+    registers (adjoint 2^12: 30.3 -> 32.1 ms; forward 2^13, 64 KiB: 11.85 -> 12.16 ms)."""
     return PERSISTENT_ADJOINT and adjoint and (1 << k) * 16 > 80 * 1024
 
 
